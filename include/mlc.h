@@ -1,0 +1,257 @@
+/*
+ * mlc.h -- C ABI of libmlc.so, the MI355X (gfx950) light-curve + chi2 path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (MLC_OK == 0) and never lets a C++ exception escape.  The text of
+ * the last failure is available from mlc_last_error(ctx).
+ *
+ * Pointer conventions: arguments named d_* are DEVICE pointers (allocated by
+ * mlc_malloc, hipMalloc or a torch tensor on the same device); every other
+ * array argument is a HOST pointer.  All work is issued on the context's
+ * stream; entry points that return host results synchronise that stream.
+ *
+ * Precision: MLC_F64 reproduces the reference arithmetic (FP64 throughout);
+ * MLC_F32 is the single-precision path (FFT, taps and per-sample terms in
+ * FP32, map mean / final per-curve reductions / magnitudes in FP64).
+ *
+ * Citations are relative to the reference tree gvernard/microSN.
+ */
+#ifndef MLC_H
+#define MLC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLC_OK 0
+#define MLC_ERR_ARG 1         /* invalid argument (checked before any launch) */
+#define MLC_ERR_HIP 2         /* HIP runtime / launch failure */
+#define MLC_ERR_ALLOC 3       /* device allocation failed */
+#define MLC_ERR_UNSUPPORTED 4 /* size or mode outside what the kernels cover */
+
+#define MLC_F32 0
+#define MLC_F64 1
+
+/* Profile shapes: the names accepted by FactoryProfile
+ * (simulating_lcurves/include/profile.hpp:142-166). */
+#define MLC_SHAPE_UNIFORM 0       /* UniformDisc  profile.cpp:185-222 */
+#define MLC_SHAPE_GAUSSIAN 1      /* Gaussian     profile.cpp:228-263 */
+#define MLC_SHAPE_GAUSSIAN_HOLE 2 /* GaussianHole profile.cpp:269-311 */
+#define MLC_SHAPE_THERMAL_HOLE 3  /* ThermalHole  profile.cpp:315-354 */
+#define MLC_SHAPE_WAVY 4          /* Wavy         profile.cpp:358-398 */
+#define MLC_SHAPE_EXPONENTIAL 5   /* Exponential (thin disc) profile.cpp:402-446 */
+
+typedef struct mlc_ctx mlc_ctx;
+typedef struct mlc_map mlc_map;
+
+/* ------------------------------------------------------------------ context
+ * Replaces the reference's implicit global CUDA state (default stream,
+ * __constant__ tables gpu_functions.cu:25-33, cuda_error util.cu:34-68). */
+int mlc_version(void);
+int mlc_ctx_create(int device, void* stream, mlc_ctx** out);
+void mlc_ctx_destroy(mlc_ctx* ctx);
+const char* mlc_last_error(const mlc_ctx* ctx);
+int mlc_ctx_set_stream(mlc_ctx* ctx, void* stream);
+int mlc_ctx_sync(mlc_ctx* ctx);
+int mlc_malloc(mlc_ctx* ctx, size_t bytes, void** d_ptr);
+int mlc_free(mlc_ctx* ctx, void* d_ptr);
+int mlc_memcpy_h2d(mlc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int mlc_memcpy_d2h(mlc_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int mlc_memset(mlc_ctx* ctx, void* d_dst, int value, size_t bytes);
+/* Per-kernel HIP-event timing on the context's stream.  mlc_timing_get
+ * synchronises and returns the summed duration and launch count of every
+ * launch of the named kernel since the last reset. */
+int mlc_timing_enable(mlc_ctx* ctx, int on);
+int mlc_timing_reset(mlc_ctx* ctx);
+int mlc_timing_get(mlc_ctx* ctx, const char* kernel, double* total_ms, long long* launches);
+/* Newline-separated names of every kernel timed since the last reset. */
+int mlc_timing_names(mlc_ctx* ctx, char* buf, size_t cap);
+
+/* ------------------------------------------------------------------ maps
+ * MagnificationMap (magnification_map.cpp:7-40): int32 ray counts ->
+ * FP64 magnification, data = counts * |avgmu/avgN| / |1/((1-k)^2-g^2)|. */
+int mlc_gerlumph_normalise(mlc_ctx* ctx, const int32_t* d_counts, int64_t npix,
+                           double avgmu, double avgN, double k, double g,
+                           double* d_map);
+
+/* Spectrum of a map: the forward 2-D real FFT of an Ny x Nx row-major FP64
+ * map (cuFFT D2Z, gpu_functions.cu:151,175,522-526), scaled by 1/(Nx*Ny) (the
+ * normalisation of kernel_multiplyFFTs, gpu_functions.cu:571-577) and held in
+ * `precision`.  With subtract_mean != 0 the FP64 mean is removed first and
+ * added back to every convolved value map (the kernel sums to one). Nx and Ny
+ * must be even, Nx/2 and Ny with prime factors in {2,3,5} only. */
+int mlc_map_create(mlc_ctx* ctx, int precision, const double* d_map, int Nx,
+                   int Ny, int subtract_mean, mlc_map** out);
+void mlc_map_destroy(mlc_map* map);
+/* Re-transform a new map of the same size into an existing handle. */
+int mlc_map_update(mlc_map* map, const double* d_map);
+
+/* Source profile on its own n x n grid, normalised to sum one
+ * (createProfileFromHalfRadius profile.hpp:142-166, createGrid/normalize
+ * profile.cpp:143-174).  rhalf and pix are used exactly as given (the
+ * reference's std::to_string/std::stof round trip is the caller's business,
+ * see mlc_parity_round).  Returns the grid size in *n; d_prof/d_dprof (FP64,
+ * capacity cap elements, either may be NULL) receive the profile and its
+ * derivative with respect to rhalf at fixed n. */
+int mlc_profile(mlc_ctx* ctx, int shape, double rhalf, double pix, double rin,
+                double a, int* n, double* d_prof, double* d_dprof, int64_t cap);
+/* The parameter round trip std::stof(std::to_string(x))
+ * (gpu_functions.cu:108,184; profile.hpp:144-160). Host only. */
+double mlc_parity_round(double x);
+/* Grid size n of a profile (host only, no device work). */
+int mlc_profile_size(int shape, double rhalf, double pix, double rin, double a);
+
+/* Convolution of the map with one profile: Kernel::setKernel
+ * (magnification_map.cpp:172-194) + D2Z of the kernel + kernel_multiplyFFTs +
+ * Z2D (gpu_functions.cu:184-200), fused: the kernel spectrum is built on the
+ * device from the n x n profile, the pointwise product is taken inside the
+ * column FFT and nothing of size Nx*Ny is staged from the host.
+ *   d_value: Ny x Nx map in the handle's precision (float or double), or NULL
+ *   d_node : Ny x Nx interleaved (value, d value / d rhalf) pairs, or NULL
+ * At least one of them must be given. */
+int mlc_map_convolve(mlc_map* map, int shape, double rhalf, double pix,
+                     double rin, double a, void* d_value, void* d_node);
+/* The same computation structured like the reference (plain path): build
+ * the full Nx x Ny kernel array on the device, forward 2-D FFT, separate
+ * multiply launch, inverse 2-D FFT, separate interleave, stream-synchronised
+ * between stages.  Used as the FP64 plain baseline. */
+int mlc_map_convolve_plain(mlc_map* map, int shape, double rhalf, double pix,
+                           double rin, double a, void* d_value, void* d_node);
+
+/* 2-D FFTs of the library (own HIP kernels), exposed for tests:
+ *   r2c: real Ny x Nx (precision) -> complex Ny x (Nx/2+1), unnormalised
+ *   c2r: complex Ny x (Nx/2+1) -> real Ny x Nx, unnormalised (input kept) */
+int mlc_fft_r2c(mlc_ctx* ctx, int precision, const void* d_in, void* d_out, int Nx, int Ny);
+int mlc_fft_c2r(mlc_ctx* ctx, int precision, const void* d_in, void* d_out, int Nx, int Ny);
+
+/* ------------------------------------------------------------------ sampling
+ * kernel_sample (gpu_functions.cu:604-620): LC[id*Nprof+k] = conv[y*Nx+x];
+ * for k > 0 also DLC[id*(Nprof-1)+k-1] = LC[id*Nprof+k] - LC[id*Nprof+k-1].
+ * conv is an Ny x Nx map in `precision`; LC/DLC are FP64. */
+int mlc_sample_points(mlc_ctx* ctx, int precision, const void* d_conv, int Nx,
+                      int Ny, int k, int Nprof, int Nloc, const int* d_loc_x,
+                      const int* d_loc_y, double* d_LC, double* d_DLC);
+
+/* expanding_source (gpu_functions.cu:82-225; gpu_functions.hpp:11): the
+ * point-source column k=0 from the raw map, then one convolution per source
+ * size sizes[1..Nprof-1] (in Einstein radii; map width `width`, Rein=1),
+ * with the reference's parameter round trip.  d_map is FP64 Ny x Nx. */
+int mlc_expanding_source(mlc_ctx* ctx, int precision, const double* d_map,
+                         int Nx, int Ny, double width, const double* sizes,
+                         int Nprof, int shape, int Nloc, const int* d_loc_x,
+                         const int* d_loc_y, double* d_LC, double* d_DLC);
+
+/* ------------------------------------------------------------------ sort
+ * Stable ascending argsort of FP64 keys by '<' (std::stable_sort semantics,
+ * the reference's thrust::stable_sort_by_key gpu_functions.cu:370-373 and
+ * its CPU twin cpu_functions.cpp:179-181).  -0.0 ties +0.0; every NaN sorts
+ * after +inf, in index order.  n < 2^32.
+ *   d_perm  : sorted position -> original index (required)
+ *   d_rank  : original index -> sorted position (the second stable sort of
+ *             gpu_functions.cu:375-377), or NULL
+ *   d_sorted: keys in sorted order, or NULL */
+int mlc_sort_f64(mlc_ctx* ctx, const double* d_keys, int64_t n, uint32_t* d_perm,
+                 uint32_t* d_rank, double* d_sorted);
+/* thrust::upper_bound + adjacent_difference (gpu_functions.cu:383-397) over
+ * an ascending array: upper[i] = #{keys <= bins[i]} (NaN never counted);
+ * lower_ind = [0, upper[0..Nbins-2]], n_per_bin = diff(upper).  Host out. */
+int mlc_bin_edges(mlc_ctx* ctx, const double* d_sorted, int64_t n,
+                  const double* bins, int Nbins, uint32_t* lower_ind,
+                  uint32_t* n_per_bin);
+
+/* ------------------------------------------------------------------ pair mode
+ * setup_integral_GPU (gpu_functions.cu:351-401; gpu_functions.hpp:15):
+ * z[a*Nloc+b] = LCA[a*Nprof]/LCB[b*Nprof] (kernel_z 643-658), stable sort,
+ * sorted_ind = rank of every pair, bin table on `bins` (host, Nbins). */
+int mlc_setup_integral(mlc_ctx* ctx, const double* d_LCA, const double* d_LCB,
+                       int Nloc, int Nprof, const double* bins, int Nbins,
+                       uint32_t* d_sorted_ind, uint32_t* lower_ind,
+                       uint32_t* n_per_bin);
+/* calculate_chi2_GPU / kernel_chi2 (gpu_functions.cu:428-456, 700-750;
+ * gpu_functions.hpp:16): chi2[sorted_ind[a*Nloc+b]] =
+ *   sum_i ((d_i - mA_i/mB_i)/s_i)^2,  mX_i = 1 if indX_i == -1 else
+ *   LCX[.,indX_i] + facX_i*DLCX[.,indX_i].   Tables are host arrays (Njp). */
+int mlc_calculate_chi2(mlc_ctx* ctx, int Njp, const int* indA, const int* indB,
+                       const double* facA, const double* facB, const double* d,
+                       const double* s, const uint32_t* d_sorted_ind,
+                       const double* d_LCA, const double* d_DLCA,
+                       const double* d_LCB, const double* d_DLCB, int Nloc,
+                       int Nprof, double* d_chi2);
+/* bin_chi2_GPU / kernel_bin_chi2 (gpu_functions.cu:483-503, 772-795;
+ * gpu_functions.hpp:17): per-bin means of exp(-chi2/2) and chi2 over the
+ * contiguous ranges [lower_ind[i], lower_ind[i]+n_per_bin[i]) of the n
+ * values of d_chi2 (n = Nloc^2, the size the reference takes from its
+ * Chi2SortBins); an empty bin gives (exp(-1000), 2000) as the reference GPU
+ * kernel does.  Host out. */
+int mlc_bin_chi2(mlc_ctx* ctx, const double* d_chi2, int64_t n,
+                 const uint32_t* lower_ind, const uint32_t* n_per_bin, int Nbins,
+                 double* binned_chi2, double* binned_exp);
+
+/* ------------------------------------------------------------------ MPD
+ * MagnificationMap::getBinnedMpd (magnification_map_gpu.cu:45-87): counts of
+ * map values v with min <= v < max on right-edge bins (host `bins`, Nbins <= 4096),
+ * value v going to the first bin whose edge is >= v; counts < 1 become
+ * 1/npix, the others count/npix.  Host out counts[Nbins]. */
+int mlc_binned_mpd(mlc_ctx* ctx, const double* d_map, int64_t npix, int Nbins,
+                   double min, double max, const double* bins, double* counts);
+
+/* ------------------------------------------------------------------ chi2_driver
+ * kernelChi2 + getChi2GPU/getChi2CPU (chi2/chi2_functions.cuh:35-190):
+ *   chi2_all[N*y+x] = sum_f sum_t ((f_obs[f,t]-mu1[x,f,t]/mu2[m,f,t])/df[f,t])^2/2
+ * FP32, with m = y (mode 0, the CPU twin; filters and epochs in one running
+ * sum) or m = x (mode 1, the reference GPU kernel at chi2_functions.cuh:52;
+ * one partial sum per filter, added in filter order).  *likelihood receives
+ * sum exp(-chi2_all) over all N^2 pairs, reduced in FP64 (host out). */
+int mlc_chi2_multiband(mlc_ctx* ctx, const float* d_f_obs, const float* d_df_obs,
+                       const float* d_mu1, const float* d_mu2, int N, int F,
+                       int T, int mode, float* d_chi2_all, double* likelihood);
+
+/* ------------------------------------------------------------------ light curves
+ * Per-curve magnitude chi2 and its adjoint along straight trajectories
+ * (generalises kernel_sample gpu_functions.cu:604-620 to bilinear taps with
+ * the pixel-centre convention of rectGrid.cpp:349-378, and the flux-ratio
+ * chi2 of kernel_chi2 700-750 to magnitudes with offsets).
+ *
+ *   d_nodes : nnodes consecutive Ny x Nx maps of interleaved (mu, dmu/drad)
+ *             in `precision` (as written by mlc_map_convolve d_node)
+ *   node_off/node_cnt (host, P*B): nodes of profile p, band b are
+ *             node_off[p*B+b] .. +node_cnt-1, radii ascending node_rad[]
+ *   band_scale (host, B): radius of band b at time t = band_scale[b]*(r0+w*t)
+ *   d_params: ncurves x 7 FP64 (x0, y0, theta, v, r0, w, dm)
+ *   d_prof, d_obs: profile and observation-set index of every curve
+ *   d_obs_off: (nobs*B+1) epoch offsets into the nepoch epochs d_t, d_mobs,
+ *             d_sig (FP64); band b of set o is [off[o*B+b], off[o*B+b+1])
+ *   out d_chi2 (ncurves, FP64) and d_grad (ncurves x 7, FP64, same order as
+ *   params).  A curve leaving the pixel-centre hull or reaching mu <= 0 gets
+ *   chi2 = +inf and a zero gradient.
+ * Between radius nodes mu is the cubic Hermite interpolant of (mu, dmu/drad);
+ * below the first / above the last node it is the first-order Taylor step. */
+int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes,
+                     int nnodes, int Nx, int Ny, int P, int B,
+                     const int* node_off, const int* node_cnt,
+                     const double* node_rad, const double* band_scale,
+                     int ncurves, const double* d_params, const int* d_prof,
+                     const int* d_obs, int nobs, const int* d_obs_off,
+                     int nepoch, const double* d_t, const double* d_mobs,
+                     const double* d_sig, double* d_chi2, double* d_grad);
+/* Same result, structured like the reference's plain pipeline: one launch
+ * writes per-sample (mu, dmu/dx, dmu/dy, dmu/drad) to device memory, the
+ * stream is synchronised, a second launch reduces chi2 and gradient.  The
+ * FP64 plain baseline. */
+int mlc_lc_chi2_grad_plain(mlc_ctx* ctx, int precision, const void* d_nodes,
+                           int nnodes, int Nx, int Ny, int P, int B,
+                           const int* node_off, const int* node_cnt,
+                           const double* node_rad, const double* band_scale,
+                           int ncurves, const double* d_params,
+                           const int* d_prof, const int* d_obs, int nobs,
+                           const int* d_obs_off, int nepoch, const double* d_t,
+                           const double* d_mobs, const double* d_sig,
+                           double* d_chi2, double* d_grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLC_H */

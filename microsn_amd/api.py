@@ -1,0 +1,365 @@
+"""Python face of libmlc.so over torch device memory.
+
+The functions mirror the reference's host interface for this path
+(simulating_lcurves/include/gpu_functions.hpp:11-18, cpu_functions.hpp:9-15,
+chi2/chi2_functions.cuh:93) with the same names, argument meaning and data
+layouts; device arrays are torch tensors on the context's device.  torch
+provides device memory and the stream; every computation runs in libmlc.so.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import F32, F64, SHAPES, MlcError, lib, ptr
+
+PREC = {"f32": F32, "f64": F64, F32: F32, F64: F64}
+
+
+def _prec(p):
+    try:
+        return PREC[p]
+    except KeyError:
+        raise ValueError(f"precision must be 'f32' or 'f64', not {p!r}") from None
+
+
+def _shape(s):
+    if isinstance(s, int):
+        return s
+    try:
+        return SHAPES[s]
+    except KeyError:
+        raise ValueError(f"unknown profile shape {s!r}") from None
+
+
+class Context:
+    """An mlc_ctx bound to one device and (by default) torch's current stream."""
+
+    def __init__(self, device=0, stream=None):
+        self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        h = C.c_void_p()
+        rc = lib().mlc_ctx_create(self.device.index or 0, C.c_void_p(stream.cuda_stream), C.byref(h))
+        if rc != 0:
+            raise MlcError(rc, "mlc_ctx_create failed")
+        self.h = h
+        self.stream = stream
+
+    def check(self, rc):
+        if rc != 0:
+            raise MlcError(rc, lib().mlc_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(lib().mlc_ctx_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mlc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- per-kernel timing (HIP events on the context stream)
+    def timing(self, on=True):
+        self.check(lib().mlc_timing_enable(self.h, int(on)))
+
+    def timing_reset(self):
+        self.check(lib().mlc_timing_reset(self.h))
+
+    def timing_get(self, name):
+        ms, n = C.c_double(), C.c_longlong()
+        self.check(lib().mlc_timing_get(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timing_all(self):
+        buf = C.create_string_buffer(1 << 16)
+        self.check(lib().mlc_timing_names(self.h, buf, len(buf)))
+        return {k: self.timing_get(k) for k in buf.value.decode().split("\n") if k}
+
+
+_default = {}
+
+
+def context(device=0):
+    """A cached Context for `device` on torch's current stream."""
+    key = (device, torch.cuda.current_stream(device).cuda_stream)
+    if key not in _default:
+        _default[key] = Context(device)
+    return _default[key]
+
+
+def _dev(ctx, t, dtype, name):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous {dtype} CUDA tensor")
+    if t.device != ctx.device:
+        raise ValueError(f"{name} is on {t.device}, the context on {ctx.device}")
+    return t
+
+
+# ------------------------------------------------------------------ maps
+class MapSpectrum:
+    """mlc_map: the map's 2-D spectrum held on the device in one precision."""
+
+    def __init__(self, ctx, dmap, precision="f32", subtract_mean=None):
+        _dev(ctx, dmap, torch.float64, "map")
+        self.ctx, self.prec = ctx, _prec(precision)
+        self.Ny, self.Nx = dmap.shape
+        if subtract_mean is None:
+            subtract_mean = self.prec == F32
+        h = C.c_void_p()
+        ctx.check(lib().mlc_map_create(ctx.h, self.prec, ptr(dmap), self.Nx, self.Ny,
+                                       int(bool(subtract_mean)), C.byref(h)))
+        self.h = h
+
+    @property
+    def dtype(self):
+        return torch.float64 if self.prec == F64 else torch.float32
+
+    def update(self, dmap):
+        _dev(self.ctx, dmap, torch.float64, "map")
+        self.ctx.check(lib().mlc_map_update(self.h, ptr(dmap)))
+
+    def convolve(self, shape, rhalf, pix=1.0, rin=0.0, a=1.0, value=True, node=False, plain=False):
+        """Convolved map (Ny, Nx) and/or node map (Ny, Nx, 2) = (mu, dmu/drhalf)."""
+        v = torch.empty((self.Ny, self.Nx), dtype=self.dtype, device=self.ctx.device) if value else None
+        nd = torch.empty((self.Ny, self.Nx, 2), dtype=self.dtype, device=self.ctx.device) if node else None
+        f = lib().mlc_map_convolve_plain if plain else lib().mlc_map_convolve
+        self.ctx.check(f(self.h, _shape(shape), float(rhalf), float(pix), float(rin), float(a),
+                         ptr(v), ptr(nd)))
+        if value and node:
+            return v, nd
+        return v if value else nd
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mlc_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gerlumph_normalise(ctx, counts, avgmu, avgN, k, g):
+    """MagnificationMap FP64 normalisation of int32 ray counts (device)."""
+    _dev(ctx, counts, torch.int32, "counts")
+    out = torch.empty(counts.shape, dtype=torch.float64, device=ctx.device)
+    ctx.check(lib().mlc_gerlumph_normalise(ctx.h, ptr(counts), counts.numel(), avgmu, avgN, k, g,
+                                           ptr(out)))
+    return out
+
+
+def profile(ctx, shape, rhalf, pix=1.0, rin=0.0, a=1.0):
+    """(profile, d profile/d rhalf), FP64 n x n device tensors."""
+    n = C.c_int()
+    ctx.check(lib().mlc_profile(ctx.h, _shape(shape), rhalf, pix, rin, a, C.byref(n), None, None, 0))
+    p = torch.empty((n.value, n.value), dtype=torch.float64, device=ctx.device)
+    dp = torch.empty_like(p)
+    ctx.check(lib().mlc_profile(ctx.h, _shape(shape), rhalf, pix, rin, a, C.byref(n), ptr(p), ptr(dp),
+                                p.numel()))
+    return p, dp
+
+
+def parity_round(x):
+    return lib().mlc_parity_round(float(x))
+
+
+def fft_r2c(ctx, x):
+    prec = F64 if x.dtype == torch.float64 else F32
+    _dev(ctx, x, x.dtype, "input")
+    Ny, Nx = x.shape
+    out = torch.empty((Ny, Nx // 2 + 1), dtype=torch.complex128 if prec == F64 else torch.complex64,
+                      device=ctx.device)
+    ctx.check(lib().mlc_fft_r2c(ctx.h, prec, ptr(x), ptr(out), Nx, Ny))
+    return out
+
+
+def fft_c2r(ctx, X, Nx):
+    prec = F64 if X.dtype == torch.complex128 else F32
+    _dev(ctx, X, X.dtype, "input")
+    Ny = X.shape[0]
+    out = torch.empty((Ny, Nx), dtype=torch.float64 if prec == F64 else torch.float32, device=ctx.device)
+    ctx.check(lib().mlc_fft_c2r(ctx.h, prec, ptr(X), ptr(out), Nx, Ny))
+    return out
+
+
+# ------------------------------------------------------------------ light curves (reference mode)
+class SimLC:
+    """data_structs.hpp:56-63: LC (Nloc x Nprof) and DLC (Nloc x Nprof-1), FP64."""
+
+    def __init__(self, ctx, Nloc, Nprof):
+        self.Nloc, self.Nprof = Nloc, Nprof
+        self.LC = torch.zeros((Nloc, Nprof), dtype=torch.float64, device=ctx.device)
+        self.DLC = torch.zeros((Nloc, max(Nprof - 1, 1)), dtype=torch.float64, device=ctx.device)
+
+
+def sample_points(ctx, conv, k, loc_x, loc_y, LC):
+    prec = F64 if conv.dtype == torch.float64 else F32
+    Ny, Nx = conv.shape
+    ctx.check(lib().mlc_sample_points(ctx.h, prec, ptr(conv), Nx, Ny, k, LC.Nprof, LC.Nloc,
+                                      ptr(loc_x), ptr(loc_y), ptr(LC.LC), ptr(LC.DLC)))
+
+
+def expanding_source(ctx, dmap, width, sizes, shape, loc_x, loc_y, LC, precision="f64"):
+    """gpu_functions.hpp:11 (expanding_source)."""
+    _dev(ctx, dmap, torch.float64, "map")
+    _dev(ctx, loc_x, torch.int32, "loc_x")
+    _dev(ctx, loc_y, torch.int32, "loc_y")
+    sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+    Ny, Nx = dmap.shape
+    ctx.check(lib().mlc_expanding_source(ctx.h, _prec(precision), ptr(dmap), Nx, Ny, float(width),
+                                         ptr(sizes), sizes.size, _shape(shape), LC.Nloc, ptr(loc_x),
+                                         ptr(loc_y), ptr(LC.LC), ptr(LC.DLC)))
+
+
+class Chi2SortBins:
+    """data_structs.hpp:41-54: sorted_ind (device, uint32 stored as int32),
+    lower_ind / n_per_bin (host uint32)."""
+
+    def __init__(self, ctx, Nloc, Nbins):
+        self.Nloc, self.Nbins = Nloc, Nbins
+        self.d_sorted_ind = torch.empty(Nloc * Nloc, dtype=torch.int32, device=ctx.device)
+        self.lower_ind = np.zeros(Nbins, np.uint32)
+        self.n_per_bin = np.zeros(Nbins, np.uint32)
+
+
+def setup_integral_GPU(ctx, LCA, LCB, bins, sort_struct):
+    """gpu_functions.hpp:15: z ratio, stable sort, sorted_ind, bin table."""
+    bins = np.ascontiguousarray(bins, np.float64)
+    ctx.check(lib().mlc_setup_integral(ctx.h, ptr(LCA.LC), ptr(LCB.LC), LCA.Nloc, LCA.Nprof,
+                                       ptr(bins), bins.size, ptr(sort_struct.d_sorted_ind),
+                                       ptr(sort_struct.lower_ind), ptr(sort_struct.n_per_bin)))
+
+
+def calculate_chi2_GPU(ctx, chi2_vars, chi2_values, sort_struct, LCA, LCB):
+    """gpu_functions.hpp:16.  chi2_vars: dict of Njp, indA, indB, facA, facB, d, s."""
+    v = chi2_vars
+    arr = {k: np.ascontiguousarray(v[k], np.int32 if k.startswith("ind") else np.float64)
+           for k in ("indA", "indB", "facA", "facB", "d", "s")}
+    ctx.check(lib().mlc_calculate_chi2(
+        ctx.h, int(v["Njp"]), ptr(arr["indA"]), ptr(arr["indB"]), ptr(arr["facA"]), ptr(arr["facB"]),
+        ptr(arr["d"]), ptr(arr["s"]), ptr(sort_struct.d_sorted_ind), ptr(LCA.LC), ptr(LCA.DLC),
+        ptr(LCB.LC), ptr(LCB.DLC), LCA.Nloc, LCA.Nprof, ptr(chi2_values)))
+
+
+def bin_chi2_GPU(ctx, sort_struct, chi2_values):
+    """gpu_functions.hpp:17 -> (binned_chi2, binned_exp) host arrays."""
+    nb = sort_struct.Nbins
+    bc, be = np.zeros(nb), np.zeros(nb)
+    ctx.check(lib().mlc_bin_chi2(ctx.h, ptr(chi2_values), chi2_values.numel(), ptr(sort_struct.lower_ind),
+                                 ptr(sort_struct.n_per_bin), nb, ptr(bc), ptr(be)))
+    return bc, be
+
+
+def sort_f64(ctx, keys, rank=False, sorted_keys=False):
+    _dev(ctx, keys, torch.float64, "keys")
+    n = keys.numel()
+    perm = torch.empty(n, dtype=torch.int32, device=ctx.device)
+    rk = torch.empty(n, dtype=torch.int32, device=ctx.device) if rank else None
+    sk = torch.empty(n, dtype=torch.float64, device=ctx.device) if sorted_keys else None
+    ctx.check(lib().mlc_sort_f64(ctx.h, ptr(keys), n, ptr(perm), ptr(rk), ptr(sk)))
+    return perm, rk, sk
+
+
+def bin_edges(ctx, sorted_keys, bins):
+    bins = np.ascontiguousarray(bins, np.float64)
+    lo = np.zeros(bins.size, np.uint32)
+    npb = np.zeros(bins.size, np.uint32)
+    ctx.check(lib().mlc_bin_edges(ctx.h, ptr(sorted_keys), sorted_keys.numel(), ptr(bins), bins.size,
+                                  ptr(lo), ptr(npb)))
+    return lo, npb
+
+
+def binned_mpd(ctx, dmap, Nbins, vmin=0.02, vmax=200.0):
+    """MagnificationMap::getBinnedMpd -> (bins, counts) host arrays."""
+    bins = mpd_bins(Nbins, vmin, vmax, True)
+    counts = np.zeros(Nbins)
+    ctx.check(lib().mlc_binned_mpd(ctx.h, ptr(dmap), dmap.numel(), Nbins, vmin, vmax, ptr(bins),
+                                   ptr(counts)))
+    return bins, counts
+
+
+def getChi2GPU(ctx, f_obs, df_obs, mu1, mu2, mode=0):
+    """chi2_functions.cuh:93 -> (chi2_all (N*N, [N*y+x]), likelihood FP64)."""
+    N, F, T = mu1.shape
+    out = torch.empty(N * N, dtype=torch.float32, device=ctx.device)
+    like = C.c_double()
+    ctx.check(lib().mlc_chi2_multiband(ctx.h, ptr(f_obs), ptr(df_obs), ptr(mu1), ptr(mu2), N, F, T,
+                                       mode, ptr(out), C.byref(like)))
+    return out, like.value
+
+
+# ------------------------------------------------------------------ host-side helpers
+def locations_on_grid(Nside, Nmap, Noff):
+    x = np.zeros(Nside * Nside, np.int32)
+    y = np.zeros_like(x)
+    if lib().mlc_locations_on_grid(Nside, Nmap, Noff, ptr(x), ptr(y)):
+        raise ValueError("bad grid arguments")
+    return x, y
+
+
+def locations_random(Nloc, Nmap, Noff, seed):
+    x = np.zeros(Nloc, np.int32)
+    y = np.zeros_like(x)
+    if lib().mlc_locations_random(Nloc, Nmap, Noff, seed, ptr(x), ptr(y)):
+        raise ValueError("bad random-location arguments")
+    return x, y
+
+
+def setup_chi2_calculation(M, V, Dfac, d, t, s, x, quirk_const_term=True):
+    """cpu_functions.hpp:10 -> dict(Njp, indA, indB, facA, facB, d, s, const_term)."""
+    d, t, s, x = (np.ascontiguousarray(a, np.float64) for a in (d, t, s, x))
+    Nd = d.size
+    indA, indB = np.zeros(Nd, np.int32), np.zeros(Nd, np.int32)
+    facA, facB, nd, ns = (np.zeros(Nd) for _ in range(4))
+    Njp, ct = C.c_int(), C.c_double()
+    rc = lib().mlc_setup_chi2(M[0], M[1], V[0], Dfac, Nd, ptr(d), ptr(t), ptr(s), x.size, ptr(x),
+                              int(quirk_const_term), ptr(indA), ptr(indB), ptr(facA), ptr(facB),
+                              ptr(nd), ptr(ns), C.byref(Njp), C.byref(ct))
+    if rc:
+        raise ValueError("bad chi2 set-up arguments")
+    n = Njp.value
+    return dict(Njp=n, indA=indA[:n], indB=indB[:n], facA=facA[:n], facB=facB[:n], d=nd[:n],
+                s=ns[:n], const_term=ct.value)
+
+
+def find_indices(t, x):
+    t, x = np.ascontiguousarray(t, np.float64), np.ascontiguousarray(x, np.float64)
+    ind = np.zeros(t.size, np.int32)
+    lib().mlc_find_indices(ptr(t), ptr(x), t.size, x.size, ptr(ind))
+    return ind
+
+
+def mpd_bins(Nbins, vmin, vmax, log_scale=True):
+    b = np.zeros(Nbins)
+    if lib().mlc_mpd_bins(Nbins, vmin, vmax, int(log_scale), ptr(b)):
+        raise ValueError("bad MPD bins")
+    return b
+
+
+def mpd_normalize(bins, counts):
+    bins = np.ascontiguousarray(bins, np.float64)
+    c = np.array(counts, np.float64)
+    lib().mlc_mpd_normalize(bins.size, ptr(bins), ptr(c))
+    return c
+
+
+def mpd_divide_by(bA, cA, bB, cB, bR):
+    bA, cA, bB, cB, bR = (np.ascontiguousarray(a, np.float64) for a in (bA, cA, bB, cB, bR))
+    out = np.zeros(bR.size)
+    if lib().mlc_mpd_divide_by(bA.size, ptr(bA), ptr(cA), bB.size, ptr(bB), ptr(cB), bR.size, ptr(bR),
+                               ptr(out)):
+        raise ValueError("bad MPDs")
+    return out
+
+
+def trapezium(f1, f2, x):
+    f1, f2, x = (np.ascontiguousarray(a, np.float64) for a in (f1, f2, x))
+    return lib().mlc_trapezium(x.size, ptr(f1), ptr(f2), ptr(x))

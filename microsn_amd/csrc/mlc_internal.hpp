@@ -1,0 +1,194 @@
+// mlc_internal.hpp -- shared host-side plumbing of libmlc.so: the context
+// (stream, last error, workspace, per-kernel event timing), status handling
+// and the launch wrapper used by every kernel file.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mlc.h"
+
+namespace mlc {
+
+// Error carried from the depths of a call to the C-ABI boundary, where it is
+// turned into a status code and the context's last-error text.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    // clear the sticky per-thread error so the next call starts clean
+    (void)hipGetLastError();
+    fail(e == hipErrorOutOfMemory ? MLC_ERR_ALLOC : MLC_ERR_HIP,
+         std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+#define MLC_REQUIRE(cond, msg)                       \
+  do {                                               \
+    if (!(cond)) ::mlc::fail(MLC_ERR_ARG, (msg));    \
+  } while (0)
+
+}  // namespace mlc
+
+struct mlc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // growable device workspace, carved into aligned slices per call
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+
+  // per-kernel timing
+  bool timing = false;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  struct Acc {
+    double ms = 0.0;
+    long long n = 0;
+  };
+  std::map<std::string, Acc> acc;
+
+  // twiddle tables: key = (n, precision) -> device array of n complex values
+  std::map<std::pair<int, int>, void*> twiddles;
+};
+
+namespace mlc {
+
+// Makes `ctx`'s device current for the lifetime of the guard.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    hip_check(hipGetDevice(&prev), "hipGetDevice");
+    if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// Slices of the context workspace.  Grows (free + malloc) when a call needs
+// more than the current size; the previous contents are not kept.
+struct Workspace {
+  mlc_ctx* ctx;
+  size_t used = 0;
+  std::vector<std::pair<size_t, size_t>> slices;  // offset, bytes
+  explicit Workspace(mlc_ctx* c) : ctx(c) {}
+  // Declare a slice; returns its index.  Call commit() before get().
+  int add(size_t bytes) {
+    size_t off = (used + 255) & ~(size_t)255;
+    slices.push_back({off, bytes});
+    used = off + bytes;
+    return (int)slices.size() - 1;
+  }
+  void commit() {
+    if (used > ctx->ws_bytes) {
+      if (ctx->ws) {
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(workspace)");
+        hip_check(hipFree(ctx->ws), "hipFree(workspace)");
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+      }
+      size_t sz = used + (used >> 3) + 4096;
+      hip_check(hipMalloc(&ctx->ws, sz), "hipMalloc(workspace)");
+      ctx->ws_bytes = sz;
+    }
+  }
+  template <class T>
+  T* get(int i) const {
+    return reinterpret_cast<T*>(static_cast<char*>(ctx->ws) + slices[i].first);
+  }
+};
+
+// Brackets one kernel launch with HIP events when timing is on.
+struct KernelTimer {
+  mlc_ctx* ctx;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelTimer(mlc_ctx* c, const char* n) : ctx(c), name(n) {
+    if (!ctx->timing) return;
+    a = take();
+    b = take();
+    hip_check(hipEventRecord(a, ctx->stream), "hipEventRecord");
+  }
+  ~KernelTimer() noexcept(false) {
+    if (!ctx->timing || !a) return;
+    hip_check(hipEventRecord(b, ctx->stream), "hipEventRecord");
+    ctx->pending.push_back({name, a, b});
+  }
+  hipEvent_t take() {
+    hipEvent_t e;
+    if (!ctx->event_pool.empty()) {
+      e = ctx->event_pool.back();
+      ctx->event_pool.pop_back();
+    } else {
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+    }
+    return e;
+  }
+};
+
+void timing_flush(mlc_ctx* ctx);  // resolve pending events into ctx->acc
+
+// Launch wrapper: timing + immediate launch-error check.
+#define MLC_LAUNCH(ctx, name, kern, grid, block, shmem, ...)                     \
+  do {                                                                           \
+    ::mlc::KernelTimer _mlc_t((ctx), (name));                                    \
+    hipLaunchKernelGGL(kern, (grid), (block), (shmem), (ctx)->stream, __VA_ARGS__); \
+    ::mlc::hip_check(hipGetLastError(), name);                                   \
+  } while (0)
+
+inline void sync(mlc_ctx* ctx) { hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }
+
+inline void h2d(mlc_ctx* ctx, void* d, const void* h, size_t bytes) {
+  if (bytes) hip_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(H2D)");
+}
+inline void d2h(mlc_ctx* ctx, void* h, const void* d, size_t bytes) {
+  if (bytes) hip_check(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync(D2H)");
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Number of workgroups that fill the device a few times over (grid-stride kernels).
+int fill_grid(mlc_ctx* ctx, int per_cu);
+
+// Turns an exception into a status code and records the message.
+template <class F>
+int guarded(mlc_ctx* ctx, F&& f) {
+  try {
+    if (!ctx) return MLC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    f();
+    return MLC_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return MLC_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return MLC_ERR_HIP;
+  } catch (...) {
+    if (ctx) ctx->err = "unknown error";
+    return MLC_ERR_HIP;
+  }
+}
+
+}  // namespace mlc

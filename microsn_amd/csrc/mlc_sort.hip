@@ -1,0 +1,363 @@
+// mlc_sort.hip -- stable LSD radix sort of FP64 keys with a 32-bit index
+// payload, and the bin table over the sorted keys.  Replaces
+// thrust::stable_sort_by_key + thrust::upper_bound + adjacent_difference
+// (gpu_functions.cu:368-397) with the ordering of std::stable_sort on '<'
+// (cpu_functions.cpp:179-181).
+//
+// Keys are mapped to uint64 so that unsigned order equals '<' order:
+// -0.0 becomes +0.0 (they compare equal), every NaN becomes the largest
+// value (NaN sorts after +inf; among NaNs the original index order is kept),
+// negative values are bit-inverted and positive ones get the sign bit set.
+// Eight 8-bit digit passes; a pass whose digit is the same for every key is
+// skipped (the all-digit histogram is taken in the transform kernel).
+//
+// One pass over a tile of 4096 keys (256 threads x 16 rounds, round r holds
+// keys r*256 + t, i.e. tile order = (round, thread)):
+//   k_tile_hist  per-tile digit counts, digit-major [256][ntiles]
+//   k_scan_tiles exclusive offsets: digit base + earlier tiles of that digit
+//   k_scatter    stable rank inside the tile (wavefront match on the 8 digit
+//                bits + per-wave counts in LDS), keys staged in LDS in digit
+//                order, then written out in runs that are contiguous in memory.
+#include <algorithm>
+#include <vector>
+
+#include "mlc_internal.hpp"
+
+using namespace mlc;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRounds = 16;
+constexpr int kTile = kThreads * kRounds;  // 4096
+constexpr int kWaves = kThreads / 64;
+
+__device__ __forceinline__ uint64_t key_of(double x) {
+  uint64_t b = (uint64_t)__double_as_longlong(x);
+  if (x != x) return ~0ull;                      // NaN
+  if (b == 0x8000000000000000ull) b = 0;         // -0.0 -> +0.0
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ unsigned digit_of(uint64_t k, int pass) {
+  return (unsigned)(k >> (8 * pass)) & 255u;
+}
+
+// Transform keys and histogram all eight digits.  hist: [8][256] uint32.
+__global__ __launch_bounds__(kThreads) void k_transform_hist(const double* __restrict__ in,
+                                                             uint64_t* __restrict__ out, int64_t n,
+                                                             unsigned* __restrict__ hist) {
+  __shared__ unsigned h[8][256];
+  for (int i = threadIdx.x; i < 8 * 256; i += kThreads) (&h[0][0])[i] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    uint64_t k = key_of(in[i]);
+    out[i] = k;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) atomicAdd(&h[p][digit_of(k, p)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * 256; i += kThreads) {
+    unsigned v = (&h[0][0])[i];
+    if (v) atomicAdd(&hist[i], v);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_hist(const uint64_t* __restrict__ keys, int64_t n,
+                                                        int pass, int ntiles,
+                                                        unsigned* __restrict__ counts) {
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    int64_t i = base + r * kThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[digit_of(keys[i], pass)], 1u);
+  }
+  __syncthreads();
+  counts[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One workgroup per digit: exclusive scan over the tiles, plus the digit's
+// global base (sum of the counts of all smaller digits for this pass).
+__global__ __launch_bounds__(1024) void k_scan_tiles(unsigned* __restrict__ counts, int ntiles,
+                                                     const unsigned* __restrict__ hist_pass) {
+  __shared__ unsigned s[1024];
+  __shared__ unsigned carry;
+  const int d = blockIdx.x, t = threadIdx.x;
+  if (t == 0) {
+    unsigned b = 0;
+    for (int j = 0; j < d; ++j) b += hist_pass[j];
+    carry = b;
+  }
+  __syncthreads();
+  unsigned* row = counts + (size_t)d * ntiles;
+  for (int start = 0; start < ntiles; start += 1024) {
+    int i = start + t;
+    unsigned v = i < ntiles ? row[i] : 0u;
+    s[t] = v;
+    __syncthreads();
+    // Hillis-Steele inclusive scan
+    for (int off = 1; off < 1024; off <<= 1) {
+      unsigned a = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += a;
+      __syncthreads();
+    }
+    unsigned c = carry;
+    if (i < ntiles) row[i] = c + s[t] - v;
+    __syncthreads();
+    if (t == 1023) carry = c + s[1023];
+    __syncthreads();
+  }
+}
+
+// Stable scatter of one pass.  payload_in == nullptr means the identity.
+__global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict__ kin,
+                                                      const unsigned* __restrict__ pin,
+                                                      uint64_t* __restrict__ kout,
+                                                      unsigned* __restrict__ pout, int64_t n,
+                                                      int pass, int ntiles,
+                                                      const unsigned* __restrict__ offsets) {
+  __shared__ uint64_t sk[kTile];
+  __shared__ unsigned sp[kTile];
+  __shared__ unsigned wave_cnt[kWaves][256];
+  __shared__ unsigned run[256];    // keys of each digit ranked so far in this tile
+  __shared__ unsigned start[256];  // first LDS slot of each digit (exclusive scan)
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  run[t] = 0;
+  for (int i = t; i < kWaves * 256; i += kThreads) (&wave_cnt[0][0])[i] = 0;
+
+  uint64_t key[kRounds];
+  unsigned pay[kRounds], rank[kRounds], dig[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    int64_t i = base + r * kThreads + t;
+    bool ok = i < n;
+    key[r] = ok ? kin[i] : ~0ull;
+    pay[r] = ok ? (pin ? pin[i] : (unsigned)i) : 0u;
+    dig[r] = ok ? digit_of(key[r], pass) : 256u;  // 256: padding, ranked last
+  }
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    const unsigned d = dig[r];
+    // lanes of this wave holding the same digit (padding lanes match nobody)
+    uint64_t peers = __ballot(d < 256u);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    if (d >= 256u) peers = 0;
+    const unsigned before = (unsigned)__popcll(peers & lt);
+    if (d < 256u && before == 0) wave_cnt[w][d] = (unsigned)__popcll(peers);
+    __syncthreads();
+    if (d < 256u) {
+      unsigned p = run[d] + before;
+      for (int ww = 0; ww < w; ++ww) p += wave_cnt[ww][d];
+      rank[r] = p;
+    }
+    __syncthreads();
+    {
+      unsigned add = 0;
+      for (int ww = 0; ww < kWaves; ++ww) {
+        add += wave_cnt[ww][t];
+        wave_cnt[ww][t] = 0;
+      }
+      run[t] += add;
+    }
+    __syncthreads();
+  }
+  // exclusive scan of the tile's digit counts (run[]) -> start[]
+  {
+    unsigned v = run[t];
+    start[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      unsigned a = t >= off ? start[t - off] : 0u;
+      __syncthreads();
+      start[t] += a;
+      __syncthreads();
+    }
+    unsigned incl = start[t];
+    __syncthreads();
+    start[t] = incl - v;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    if (dig[r] < 256u) {
+      unsigned slot = start[dig[r]] + rank[r];
+      sk[slot] = key[r];
+      sp[slot] = pay[r];
+    }
+  }
+  __syncthreads();
+  const int valid = (int)min<int64_t>(kTile, n - base);
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    int slot = r * kThreads + t;
+    if (slot < valid) {
+      uint64_t k = sk[slot];
+      unsigned d = digit_of(k, pass);
+      size_t dst = (size_t)offsets[(size_t)d * ntiles + blockIdx.x] + (slot - start[d]);
+      kout[dst] = k;
+      pout[dst] = sp[slot];
+    }
+  }
+}
+
+__global__ void k_iota(unsigned* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (unsigned)i;
+}
+
+__global__ void k_rank_and_gather(const unsigned* __restrict__ perm, const double* __restrict__ keys,
+                                  int64_t n, unsigned* __restrict__ rank, double* __restrict__ sorted) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    unsigned p = perm[i];
+    if (rank) rank[p] = (unsigned)i;
+    if (sorted) sorted[i] = keys[p];
+  }
+}
+
+// upper[i] = #{non-NaN keys <= bins[i]} of an ascending array (NaNs last).
+__global__ void k_upper_bound(const double* __restrict__ s, int64_t n, const double* __restrict__ bins,
+                              int nb, unsigned* __restrict__ upper) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  // number of non-NaN keys
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    double v = s[mid];
+    if (v != v) hi = mid;
+    else lo = mid + 1;
+  }
+  const int64_t nvalid = lo;
+  const double b = bins[i];
+  lo = 0;
+  hi = nvalid;
+  while (lo < hi) {  // first index with b < s[idx]
+    int64_t mid = (lo + hi) >> 1;
+    if (b < s[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  upper[i] = (unsigned)lo;
+}
+
+}  // namespace
+
+namespace mlc {
+
+// Stable argsort into d_perm (sorted position -> original index).
+void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsigned* rank,
+              double* sorted) {
+  if (n <= 0) return;
+  MLC_REQUIRE(n < ((int64_t)1 << 32), "sort: more than 2^32-1 keys");
+  const int ntiles = (int)((n + kTile - 1) / kTile);
+  Workspace ws(ctx);
+  int ik0 = ws.add(sizeof(uint64_t) * n), ik1 = ws.add(sizeof(uint64_t) * n);
+  int ip1 = ws.add(sizeof(unsigned) * n);
+  int ih = ws.add(sizeof(unsigned) * 8 * 256);
+  int ic = ws.add(sizeof(unsigned) * 256 * (size_t)ntiles);
+  ws.commit();
+  uint64_t *k0 = ws.get<uint64_t>(ik0), *k1 = ws.get<uint64_t>(ik1);
+  unsigned *pbuf = ws.get<unsigned>(ip1), *hist = ws.get<unsigned>(ih), *cnt = ws.get<unsigned>(ic);
+  hip_check(hipMemsetAsync(hist, 0, sizeof(unsigned) * 8 * 256, ctx->stream), "hipMemsetAsync(hist)");
+  const int g = std::min<int64_t>(cdiv(n, kThreads), fill_grid(ctx, 4));
+  MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
+             hist);
+  std::vector<unsigned> h(8 * 256);
+  d2h(ctx, h.data(), hist, sizeof(unsigned) * h.size());
+  sync(ctx);
+  std::vector<int> passes;
+  for (int p = 0; p < 8; ++p) {
+    bool trivial = false;
+    for (int d = 0; d < 256; ++d)
+      if (h[p * 256 + d] == (unsigned)n) trivial = true;
+    if (!trivial) passes.push_back(p);
+  }
+  if (passes.empty()) {
+    MLC_LAUNCH(ctx, "sort_iota", k_iota, dim3(std::min<int64_t>(cdiv(n, 256), 4096)), dim3(256), 0,
+               perm, n);
+  } else {
+    // ping-pong so that the last pass writes its payload into perm
+    const int np = (int)passes.size();
+    unsigned* pa = (np % 2 == 1) ? perm : pbuf;  // output of pass 0
+    unsigned* pb = (np % 2 == 1) ? pbuf : perm;
+    const uint64_t* kin = k0;
+    uint64_t* kout = k1;
+    const unsigned* pin = nullptr;
+    unsigned* pout = pa;
+    for (int q = 0; q < np; ++q) {
+      const int p = passes[q];
+      MLC_LAUNCH(ctx, "sort_tile_hist", k_tile_hist, dim3(ntiles), dim3(kThreads), 0, kin, n, p,
+                 ntiles, cnt);
+      MLC_LAUNCH(ctx, "sort_scan_tiles", k_scan_tiles, dim3(256), dim3(1024), 0, cnt, ntiles,
+                 hist + p * 256);
+      MLC_LAUNCH(ctx, "sort_scatter", k_scatter, dim3(ntiles), dim3(kThreads), 0, kin, pin, kout,
+                 pout, n, p, ntiles, cnt);
+      // next pass reads what this one wrote
+      const uint64_t* nk = kout;
+      kout = (kout == k1) ? k0 : k1;
+      kin = nk;
+      pin = pout;
+      pout = (pout == pa) ? pb : pa;
+    }
+  }
+  if (rank || sorted)
+    MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
+               dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, keys,
+               n, rank, sorted);
+}
+
+void bin_edges(mlc_ctx* ctx, const double* sorted, int64_t n, const double* bins, int Nbins,
+               unsigned* lower_ind, unsigned* n_per_bin) {
+  Workspace ws(ctx);
+  int ib = ws.add(sizeof(double) * Nbins), iu = ws.add(sizeof(unsigned) * Nbins);
+  ws.commit();
+  double* db = ws.get<double>(ib);
+  unsigned* du = ws.get<unsigned>(iu);
+  h2d(ctx, db, bins, sizeof(double) * Nbins);
+  MLC_LAUNCH(ctx, "bin_upper_bound", k_upper_bound, dim3(cdiv(Nbins, 64)), dim3(64), 0, sorted, n,
+             db, Nbins, du);
+  std::vector<unsigned> up(Nbins);
+  d2h(ctx, up.data(), du, sizeof(unsigned) * Nbins);
+  sync(ctx);
+  lower_ind[0] = 0;
+  for (int i = 1; i < Nbins; ++i) lower_ind[i] = up[i - 1];
+  n_per_bin[0] = up[0];
+  for (int i = 1; i < Nbins; ++i) n_per_bin[i] = up[i] - up[i - 1];
+}
+
+}  // namespace mlc
+
+extern "C" {
+
+int mlc_sort_f64(mlc_ctx* ctx, const double* d_keys, int64_t n, uint32_t* d_perm, uint32_t* d_rank,
+                 double* d_sorted) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(n >= 0, "mlc_sort_f64: negative n");
+    MLC_REQUIRE(n == 0 || (d_keys && d_perm), "mlc_sort_f64: null pointer");
+    sort_f64(ctx, d_keys, n, d_perm, d_rank, d_sorted);
+  });
+}
+
+int mlc_bin_edges(mlc_ctx* ctx, const double* d_sorted, int64_t n, const double* bins, int Nbins,
+                  uint32_t* lower_ind, uint32_t* n_per_bin) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(Nbins > 0 && bins && lower_ind && n_per_bin, "mlc_bin_edges: bad bins");
+    MLC_REQUIRE(n >= 0 && (n == 0 || d_sorted), "mlc_bin_edges: bad keys");
+    bin_edges(ctx, d_sorted, n, bins, Nbins, lower_ind, n_per_bin);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+"""Light curves along straight trajectories with a per-curve magnitude chi2
+and its adjoint (libmlc.so: mlc_map_convolve node maps + mlc_lc_chi2_grad),
+and the torch.autograd.Function that exposes them to a Python sampler (the
+caller slot of driver_code/sampling.ipynb).
+
+Curve parameters, one row of 7 per curve:
+    x0, y0      trajectory origin (pixels; pixel centres at integer coords)
+    theta       direction (radians)
+    v           effective velocity (pixels per time unit)
+    r0, w       source half-light radius at t = 0 and its growth rate:
+                radius of band b at time t = band_scale[b] * (r0 + w t)
+    dm          magnitude offset of the image
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import F32, F64, lib, ptr
+from .api import MapSpectrum, _dev, _prec
+
+NPARAM = 7
+
+
+class NodeSet:
+    """Convolved (mu, dmu/drhalf) maps at a few radii per (profile, band).
+
+    nodes[k] is the node map k, shape (Ny, Nx, 2); table[p][b] lists the node
+    indices of profile p in band b with ascending radii."""
+
+    def __init__(self, ctx, spectrum: MapSpectrum, shapes, radii, band_scale, pix=1.0, plain=False):
+        """shapes: P profile shapes; radii[p][b]: ascending list of node radii
+        (half-light radius in pixels) for profile p, band b."""
+        self.ctx = ctx
+        self.P, self.B = len(shapes), len(band_scale)
+        self.Ny, self.Nx = spectrum.Ny, spectrum.Nx
+        self.prec = spectrum.prec
+        rad, off, cnt = [], [], []
+        for p in range(self.P):
+            for b in range(self.B):
+                rs = list(radii[p][b])
+                off.append(len(rad))
+                cnt.append(len(rs))
+                rad.extend(float(r) for r in rs)
+        self.node_off = np.array(off, np.int32)
+        self.node_cnt = np.array(cnt, np.int32)
+        self.node_rad = np.array(rad, np.float64)
+        self.band_scale = np.ascontiguousarray(band_scale, np.float64)
+        self.nodes = torch.empty((len(rad), self.Ny, self.Nx, 2), dtype=spectrum.dtype, device=ctx.device)
+        k = 0
+        for p in range(self.P):
+            for b in range(self.B):
+                for r in radii[p][b]:
+                    f = lib().mlc_map_convolve_plain if plain else lib().mlc_map_convolve
+                    ctx.check(f(spectrum.h, _shape_id(shapes[p]), float(r), float(pix), 0.0, 1.0, None,
+                                ptr(self.nodes[k])))
+                    k += 1
+
+
+def _shape_id(s):
+    from ._lib import SHAPES
+    return s if isinstance(s, int) else SHAPES[s]
+
+
+class Observations:
+    """Observed epochs of nobs observation sets: for set o and band b the
+    epochs are t[off[o*B+b] : off[o*B+b+1]] (FP64 on the device)."""
+
+    def __init__(self, ctx, B, sets):
+        """sets: list of per-set lists of B (t, mobs, sig) triples of arrays."""
+        t, m, s, off = [], [], [], [0]
+        for bands in sets:
+            assert len(bands) == B
+            for (tb, mb, sb) in bands:
+                tb, mb, sb = (np.atleast_1d(np.asarray(a, np.float64)) for a in (tb, mb, sb))
+                assert tb.shape == mb.shape == sb.shape
+                t.append(tb)
+                m.append(mb)
+                s.append(sb)
+                off.append(off[-1] + tb.size)
+        self.B, self.nobs = B, len(sets)
+        cat = lambda xs: torch.tensor(np.concatenate(xs) if xs else np.zeros(0), dtype=torch.float64,
+                                      device=ctx.device)
+        self.t, self.mobs, self.sig = cat(t), cat(m), cat(s)
+        self.off = torch.tensor(np.array(off, np.int32), device=ctx.device)
+        self.nepoch = int(off[-1])
+
+
+def chi2_grad(ctx, nodes: NodeSet, obs: Observations, params, prof, obs_idx, plain=False):
+    """(chi2 [n], grad [n, 7]) FP64 device tensors for n curves."""
+    _dev(ctx, params, torch.float64, "params")
+    _dev(ctx, prof, torch.int32, "prof")
+    _dev(ctx, obs_idx, torch.int32, "obs")
+    n = params.shape[0]
+    assert params.shape == (n, NPARAM) and prof.shape == (n,) and obs_idx.shape == (n,)
+    chi2 = torch.empty(n, dtype=torch.float64, device=ctx.device)
+    grad = torch.empty((n, NPARAM), dtype=torch.float64, device=ctx.device)
+    f = lib().mlc_lc_chi2_grad_plain if plain else lib().mlc_lc_chi2_grad
+    ctx.check(f(ctx.h, nodes.prec, ptr(nodes.nodes), nodes.nodes.shape[0], nodes.Nx, nodes.Ny, nodes.P,
+                nodes.B, ptr(nodes.node_off), ptr(nodes.node_cnt), ptr(nodes.node_rad),
+                ptr(nodes.band_scale), n, ptr(params), ptr(prof), ptr(obs_idx), obs.nobs, ptr(obs.off),
+                obs.nepoch, ptr(obs.t), ptr(obs.mobs), ptr(obs.sig), ptr(chi2), ptr(grad)))
+    return chi2, grad
+
+
+class LightCurveChi2(torch.autograd.Function):
+    """chi2 per curve as a differentiable function of the curve parameters.
+
+    forward runs the fused chi2 + adjoint kernel once and keeps the gradient;
+    backward scales it by the incoming gradient."""
+
+    @staticmethod
+    def forward(fctx, params, ctx, nodes, obs, prof, obs_idx):
+        chi2, grad = chi2_grad(ctx, nodes, obs, params.detach().contiguous(), prof, obs_idx)
+        fctx.save_for_backward(grad)
+        return chi2
+
+    @staticmethod
+    def backward(fctx, g):
+        (grad,) = fctx.saved_tensors
+        return g[:, None] * grad, None, None, None, None, None
+
+
+def lightcurve_chi2(params, ctx, nodes, obs, prof, obs_idx):
+    return LightCurveChi2.apply(params, ctx, nodes, obs, prof, obs_idx)

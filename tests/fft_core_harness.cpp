@@ -1,0 +1,100 @@
+// fft_core_harness.cpp -- CPU driver for the butterflies, Stockham index map
+// and real-FFT pre/post steps of microsn_amd/csrc/mlc_fft_core.hpp (the same
+// header the gfx950 kernels include).  Built by tests/test_fft_core.py with
+// g++ and compared against numpy's FFT.  Test infrastructure only.
+#include <cmath>
+#include <vector>
+
+#include "../microsn_amd/csrc/mlc_fft_core.hpp"
+
+using namespace mlc;
+
+template <class T, bool INV>
+static void run(std::vector<cpx<T>>& a, int L) {
+  Plan p;
+  if (!make_plan(L, &p)) return;
+  std::vector<cpx<T>> tw(L), b(L);
+  for (int m = 0; m < L; ++m) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / L;
+    tw[m] = mk<T>((T)std::cos(ang), (T)std::sin(ang));
+  }
+  int Ns = 1;
+  for (int s = 0; s < p.n; ++s) {
+    int R = p.r[s];
+    for (int j = 0; j < L / R; ++j) {
+      cpx<T> v[8];
+      switch (R) {
+        case 2: stage_load<T, 2, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 2>(b.data(), Ns, j, v); break;
+        case 3: stage_load<T, 3, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 3>(b.data(), Ns, j, v); break;
+        case 4: stage_load<T, 4, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 4>(b.data(), Ns, j, v); break;
+        case 5: stage_load<T, 5, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 5>(b.data(), Ns, j, v); break;
+        case 8: stage_load<T, 8, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 8>(b.data(), Ns, j, v); break;
+      }
+    }
+    a.swap(b);
+    Ns *= R;
+  }
+}
+
+extern "C" {
+
+// In-place complex FFT of n interleaved doubles; returns -1 for an
+// unsupported length.
+int h_fft(double* data, int L, int inverse) {
+  Plan p;
+  if (!make_plan(L, &p)) return -1;
+  std::vector<cpx<double>> a(L);
+  for (int i = 0; i < L; ++i) a[i] = mk(data[2 * i], data[2 * i + 1]);
+  if (inverse) run<double, true>(a, L);
+  else run<double, false>(a, L);
+  for (int i = 0; i < L; ++i) {
+    data[2 * i] = a[i].x;
+    data[2 * i + 1] = a[i].y;
+  }
+  return 0;
+}
+
+// Real forward FFT of N reals -> N/2+1 complex (interleaved), through the
+// half-length complex transform and r2c_post.
+int h_rfft(const double* x, int N, double* X) {
+  int M = N / 2;
+  std::vector<cpx<double>> z(M);
+  for (int n = 0; n < M; ++n) z[n] = mk(x[2 * n], x[2 * n + 1]);
+  Plan p;
+  if (N % 2 || !make_plan(M, &p)) return -1;
+  run<double, false>(z, M);
+  for (int k = 0; k <= M / 2; ++k) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * k / N;
+    cpx<double> w = mk((double)std::cos(ang), (double)std::sin(ang));
+    cpx<double> Xk, Xm;
+    r2c_post(z[k % M], z[(M - k) % M], w, &Xk, &Xm);
+    X[2 * k] = Xk.x;
+    X[2 * k + 1] = Xk.y;
+    X[2 * (M - k)] = Xm.x;
+    X[2 * (M - k) + 1] = Xm.y;
+  }
+  return 0;
+}
+
+// Unnormalised inverse real FFT: N/2+1 complex -> N reals.
+int h_irfft(const double* X, int N, double* x) {
+  int M = N / 2;
+  Plan p;
+  if (N % 2 || !make_plan(M, &p)) return -1;
+  std::vector<cpx<double>> z(M);
+  for (int k = 0; k <= M / 2; ++k) {
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * k / N;
+    cpx<double> w = mk((double)std::cos(ang), (double)std::sin(ang));
+    cpx<double> Zk, Zm;
+    c2r_pre(mk(X[2 * k], X[2 * k + 1]), mk(X[2 * (M - k)], X[2 * (M - k) + 1]), w, &Zk, &Zm);
+    z[k] = Zk;
+    if (k > 0) z[M - k] = Zm;
+  }
+  run<double, true>(z, M);
+  for (int n = 0; n < M; ++n) {
+    x[2 * n] = z[n].x;
+    x[2 * n + 1] = z[n].y;
+  }
+  return 0;
+}
+}

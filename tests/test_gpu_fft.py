@@ -1,0 +1,122 @@
+"""GPU parity of the library's 2-D FFT and of the map x profile convolution
+(cuFFT D2Z/Z2D + kernel_multiplyFFTs + setKernel, gpu_functions.cu:151-200).
+
+cuFFT itself pins nothing (no reference test holds its outputs), so the
+transforms are checked against numpy's FFT in FP64 and the convolution
+against the oracle's FP64 FFT convolution and direct circular sums."""
+import numpy as np
+import pytest
+import torch
+
+from microsn_amd import api
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(8, 2), (16, 16), (48, 64), (60, 100), (250, 256), (1000, 1000), (96, 4096), (4096, 40)]
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("Ny,Nx", SIZES)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_r2c_c2r_match_numpy(ctx, Ny, Nx, prec):
+    rng = np.random.default_rng(Nx * 7 + Ny)
+    x = rng.standard_normal((Ny, Nx))
+    dt = torch.float64 if prec == "f64" else torch.float32
+    X = api.fft_r2c(ctx, torch.tensor(x, dtype=dt, device="cuda")).cpu().numpy()
+    ref = np.fft.rfft2(x)
+    tol = 1e-13 if prec == "f64" else 3e-6
+    assert _rel(X, ref) < tol
+    # inverse: unnormalised, input not modified
+    Xin = torch.tensor(ref, dtype=torch.complex128 if prec == "f64" else torch.complex64, device="cuda")
+    keep = Xin.clone()
+    y = api.fft_c2r(ctx, Xin, Nx).cpu().numpy()
+    assert torch.equal(Xin, keep)
+    assert _rel(y, x * Nx * Ny) < tol
+
+
+def _smooth_map(rng, Ny, Nx):
+    g = rng.standard_normal((Ny, Nx))
+    return 5.0 * np.exp(0.5 * g) / np.exp(0.125)
+
+
+@pytest.mark.parametrize("shape", ["gaussian", "uniform", "exponential", "gaussian_hole",
+                                   "thermal_hole", "wavy"])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_convolve_matches_oracle(ctx, orc, shape, prec):
+    rng = np.random.default_rng(3)
+    Ny, Nx = 64, 80
+    m = _smooth_map(rng, Ny, Nx)
+    rhalf, pix, rin, a = 4.3, 1.0, 1.5, 1.0
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
+    val, node = spec.convolve(shape, rhalf, pix, rin=rin, a=a, value=True, node=True)
+    prof, dprof = orc.profile(shape, rhalf, pix, rin=rin, a=a, deriv=True)
+    ref = orc.fft_convolve(m, orc.set_kernel(prof, Nx, Ny))
+    dref = orc.fft_convolve(m, orc.set_kernel(dprof, Nx, Ny))
+    tol = 1e-12 if prec == "f64" else 2e-6
+    v = val.cpu().numpy().astype(np.float64)
+    nd = node.cpu().numpy().astype(np.float64)
+    assert _rel(v, ref) < tol
+    assert _rel(nd[..., 0], ref) < tol
+    # the derivative map has zero mean; compare against its own scale
+    assert np.max(np.abs(nd[..., 1] - dref)) < tol * 20 * max(np.max(np.abs(ref)), 1.0)
+    # plain (reference-structured) path gives the same maps
+    pv, pn = spec.convolve(shape, rhalf, pix, rin=rin, a=a, value=True, node=True, plain=True)
+    assert _rel(pv.cpu().numpy().astype(np.float64), ref) < tol
+    assert np.max(np.abs(pn.cpu().numpy()[..., 1].astype(np.float64) - dref)) < tol * 20 * np.max(np.abs(ref))
+
+
+def test_convolve_direct_sum_and_identity(ctx, orc):
+    """Direct circular convolution on a tiny map, and the mean preserved."""
+    rng = np.random.default_rng(11)
+    m = _smooth_map(rng, 24, 30)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), "f64")
+    v = spec.convolve("gaussian", 2.2).cpu().numpy()
+    prof = orc.profile("gaussian", 2.2)
+    direct = orc.direct_convolve(m, orc.set_kernel(prof, 30, 24))
+    assert _rel(v, direct) < 1e-12
+    assert abs(v.mean() - m.mean()) < 1e-12 * m.mean()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_convolve_4096_properties(ctx, prec):
+    """Full-size map: mean preserved and a translated impulse lands where the
+    wrapped kernel puts it (the +1/2 px corner convention of setKernel)."""
+    N = 4096
+    m = torch.full((N, N), 2.0, dtype=torch.float64, device="cuda")
+    m[1000, 2000] += 1000.0
+    spec = api.MapSpectrum(ctx, m, prec)
+    v = spec.convolve("gaussian", 5.0).double()
+    mean = float(v.mean())
+    assert abs(mean - float(m.mean())) < (1e-12 if prec == "f64" else 1e-6) * float(m.mean())
+    # centroid of the excess around the impulse: offset (-1/2, -1/2) px from
+    # kernel[dy][dx] = P((dx+1/2) pix, (dy+1/2) pix)
+    w = (v - 2.0)[980:1021, 1980:2021]
+    yy, xx = torch.meshgrid(torch.arange(980, 1021, device="cuda", dtype=torch.float64),
+                            torch.arange(1980, 2021, device="cuda", dtype=torch.float64), indexing="ij")
+    cy = float((w * yy).sum() / w.sum())
+    cx = float((w * xx).sum() / w.sum())
+    assert abs(cy - 999.5) < 1e-3 and abs(cx - 1999.5) < 1e-3
+
+
+def test_profile_on_device_matches_oracle(ctx, orc):
+    for shape in ["gaussian", "uniform", "exponential", "gaussian_hole", "thermal_hole", "wavy"]:
+        p, dp = api.profile(ctx, shape, 6.7, 1.0, rin=2.0, a=1.0)
+        rp, rdp = orc.profile(shape, 6.7, 1.0, rin=2.0, a=1.0, deriv=True)
+        assert p.shape == rp.shape
+        assert np.max(np.abs(p.cpu().numpy() - rp)) < 1e-14
+        assert np.max(np.abs(dp.cpu().numpy() - rdp)) < 1e-13
+
+
+def test_unsupported_sizes_rejected(ctx):
+    m = torch.ones((14, 22), dtype=torch.float64, device="cuda")  # 11 = prime
+    with pytest.raises(Exception, match="prime factor"):
+        api.MapSpectrum(ctx, m, "f32")
+    m = torch.ones((16, 15), dtype=torch.float64, device="cuda")  # odd width
+    with pytest.raises(Exception, match="even"):
+        api.MapSpectrum(ctx, m, "f32")
+    spec = api.MapSpectrum(ctx, torch.ones((32, 32), dtype=torch.float64, device="cuda"), "f32")
+    with pytest.raises(Exception, match="larger than the map"):
+        spec.convolve("gaussian", 40.0)

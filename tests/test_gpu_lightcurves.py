@@ -1,0 +1,158 @@
+"""GPU parity of the trajectory light curves, per-curve magnitude chi2 and
+adjoint (mlc_map_convolve node maps + mlc_lc_chi2_grad) against the FP64
+oracle, central finite differences and the FP64 path."""
+import numpy as np
+import pytest
+import torch
+
+from microsn_amd import api
+from microsn_amd import lightcurves as lcm
+
+pytestmark = pytest.mark.gpu
+
+
+def _map(rng, Ny, Nx):
+    g = rng.standard_normal((Ny, Nx))
+    return 5.0 * np.exp(0.5 * g) / np.exp(0.125)
+
+
+def _setup(ctx, prec, rng, Ny=128, Nx=160, radii=((2.0, 3.0, 4.5), (2.5,)), shapes=("gaussian", "exponential"),
+           band_scale=(1.0, 1.5), smooth=True):
+    m = _map(rng, Ny, Nx)
+    if smooth:  # smoothed field, as the synthetic configurations use
+        s0 = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), "f64")
+        m = s0.convolve("gaussian", 3.0 * 1.18).cpu().numpy()
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
+    P, B = len(shapes), len(band_scale)
+    rr = [[list(radii[(p + b) % len(radii)]) for b in range(B)] for p in range(P)]
+    nodes = lcm.NodeSet(ctx, spec, shapes, rr, band_scale)
+    sets = []
+    for o in range(3):
+        bands = []
+        for b in range(B):
+            ne = int(rng.integers(1 if o else 0, 40))
+            t = np.sort(rng.uniform(0, 30, ne))
+            bands.append((t, rng.normal(-1.7, 0.3, ne), rng.uniform(0.03, 0.08, ne)))
+        sets.append(bands)
+    obs = lcm.Observations(ctx, B, sets)
+    return m, spec, nodes, obs, sets
+
+
+def _params(rng, n, Ny, Nx, P, nobs, tmax=30.0):
+    L = 0.9 * tmax * 1.5
+    x0 = rng.uniform(L, Nx - 1 - L, n)
+    y0 = rng.uniform(L, Ny - 1 - L, n) if Ny - 1 - 2 * L > 0 else np.full(n, (Ny - 1) / 2)
+    th = rng.uniform(0, 2 * np.pi, n)
+    v = rng.uniform(0.2, 1.0, n)
+    r0 = rng.uniform(2.0, 3.5, n)
+    w = rng.uniform(0.0, 0.04, n)
+    dm = rng.normal(0, 0.1, n)
+    prm = np.stack([x0, y0, th, v, r0, w, dm], 1)
+    return prm, rng.integers(0, P, n).astype(np.int32), rng.integers(0, nobs, n).astype(np.int32)
+
+
+def _oracle(orc, nodes, sets, prm, prof, obsi):
+    maps = nodes.nodes.double().cpu().numpy()
+    off = [0]
+    t, mo, sg = [], [], []
+    for bands in sets:
+        for (tb, mb, sb) in bands:
+            t.append(tb); mo.append(mb); sg.append(sb)
+            off.append(off[-1] + len(tb))
+    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)
+    return orc.lc_chi2_grad(maps, nodes.B, nodes.node_off, nodes.node_cnt, nodes.node_rad, None,
+                            nodes.band_scale, prm, prof, obsi, np.array(off, np.int32), cat(t), cat(mo),
+                            cat(sg))
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_f64_matches_oracle(ctx, orc, plain):
+    rng = np.random.default_rng(10)
+    m, spec, nodes, obs, sets = _setup(ctx, "f64", rng)
+    prm, prof, obsi = _params(rng, 300, 128, 160, nodes.P, obs.nobs)
+    prm[0, 0] = -5.0  # leaves the map: +inf, zero gradient
+    c, g = lcm.chi2_grad(ctx, nodes, obs, torch.tensor(prm, device="cuda"), torch.tensor(prof, device="cuda"),
+                         torch.tensor(obsi, device="cuda"), plain=plain)
+    rc, rg = _oracle(orc, nodes, sets, prm, prof, obsi)
+    c, g = c.cpu().numpy(), g.cpu().numpy()
+    assert np.isinf(c[0]) and np.isinf(rc[0]) and not g[0].any()
+    fin = np.isfinite(rc)
+    assert np.array_equal(np.isfinite(c), fin)
+    assert np.max(np.abs(c[fin] - rc[fin]) / np.maximum(np.abs(rc[fin]), 1e-30)) < 1e-11
+    scale = np.maximum(np.abs(rg[fin]).max(axis=1, keepdims=True), 1e-30)
+    assert np.max(np.abs(g[fin] - rg[fin]) / scale) < 1e-10
+
+
+def test_adjoint_matches_central_differences(ctx):
+    rng = np.random.default_rng(12)
+    m, spec, nodes, obs, sets = _setup(ctx, "f64", rng)
+    prm, prof, obsi = _params(rng, 8, 128, 160, nodes.P, obs.nobs)
+    obsi[:] = 1
+    P = torch.tensor(prm, device="cuda")
+    pr, ob = torch.tensor(prof, device="cuda"), torch.tensor(obsi, device="cuda")
+    c0, g0 = lcm.chi2_grad(ctx, nodes, obs, P, pr, ob)
+    h = np.array([1e-6, 1e-6, 1e-7, 1e-7, 1e-6, 1e-7, 1e-6])
+    for q in range(7):
+        Pp, Pm = P.clone(), P.clone()
+        Pp[:, q] += h[q]
+        Pm[:, q] -= h[q]
+        cp, _ = lcm.chi2_grad(ctx, nodes, obs, Pp, pr, ob)
+        cm, _ = lcm.chi2_grad(ctx, nodes, obs, Pm, pr, ob)
+        fd = ((cp - cm) / (2 * h[q])).cpu().numpy()
+        ad = g0[:, q].cpu().numpy()
+        err = np.abs(fd - ad) / np.maximum(np.abs(ad), 1e-3 * np.abs(g0.cpu().numpy()).max())
+        assert np.median(err) < 1e-4 and np.max(err) < 2e-2, (q, fd, ad)
+
+
+def test_node_derivative_matches_convolution_fd(ctx):
+    """The d/drhalf node map equals central differences of the convolved map
+    in rhalf at a fixed grid size n (Gaussian: n = ceil(4 rhalf) stays 18)."""
+    rng = np.random.default_rng(13)
+    m = _map(rng, 96, 96)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), "f64")
+    r, h = 4.3, 1e-5
+    node = spec.convolve("gaussian", r, value=False, node=True)
+    vp = spec.convolve("gaussian", r + h)
+    vm = spec.convolve("gaussian", r - h)
+    fd = ((vp - vm) / (2 * h)).cpu().numpy()
+    d = node[..., 1].cpu().numpy()
+    assert np.max(np.abs(fd - d)) < 1e-7 * np.max(np.abs(d))
+
+
+def test_f32_chi2_within_1e5_of_f64(ctx):
+    """Three profiles at three bands with radii 2/4/8 px (one node each, the
+    radius sits on the node): FP32 maps + FP32 taps against FP64."""
+    rng = np.random.default_rng(14)
+    shapes, bs = ("gaussian", "uniform", "exponential"), (1.0, 2.0, 4.0)
+    sm, s64, _, obs, sets = _setup(ctx, "f64", np.random.default_rng(14), Ny=512, Nx=512,
+                                   radii=((2.0,),), shapes=shapes, band_scale=bs)
+    s32 = api.MapSpectrum(ctx, torch.tensor(sm, device="cuda"), "f32")
+    rr = [[[2.0], [4.0], [8.0]] for _ in range(3)]
+    n64 = lcm.NodeSet(ctx, s64, shapes, rr, bs)
+    n32 = lcm.NodeSet(ctx, s32, shapes, rr, bs)
+    prm, prof, obsi = _params(rng, 2000, 512, 512, 3, obs.nobs)
+    prm[:, 4] = 2.0
+    prm[:, 5] = 0.0
+    args = [torch.tensor(a, device="cuda") for a in (prm, prof, obsi)]
+    c64, g64 = lcm.chi2_grad(ctx, n64, obs, *args)
+    c32, g32 = lcm.chi2_grad(ctx, n32, obs, *args)
+    fin = torch.isfinite(c64) & (c64 > 0)
+    rel = ((c32 - c64).abs() / c64.abs())[fin]
+    assert float(rel.max()) <= 1e-5, float(rel.max())
+    # light-curve error in magnitudes: node maps themselves
+    mag = (2.5 / np.log(10)) * ((n32.nodes[..., 0].double() - n64.nodes[..., 0]).abs() / n64.nodes[..., 0])
+    assert float(mag.max()) < 1e-5
+
+
+def test_autograd_function(ctx):
+    rng = np.random.default_rng(15)
+    m, spec, nodes, obs, sets = _setup(ctx, "f32", rng)
+    prm, prof, obsi = _params(rng, 64, 128, 160, nodes.P, obs.nobs)
+    P = torch.tensor(prm, device="cuda", requires_grad=True)
+    pr, ob = torch.tensor(prof, device="cuda"), torch.tensor(obsi, device="cuda")
+    c = lcm.lightcurve_chi2(P, ctx, nodes, obs, pr, ob)
+    w = torch.linspace(0.5, 1.5, 64, device="cuda", dtype=torch.float64)
+    (c * w).sum().backward()
+    c2, g2 = lcm.chi2_grad(ctx, nodes, obs, P.detach(), pr, ob)
+    assert torch.equal(c.detach(), c2)
+    assert torch.allclose(P.grad, w[:, None] * g2, rtol=0, atol=0)
