@@ -1,0 +1,291 @@
+"""Light curves/s with chi2 + gradient at a 4096x4096 map (MI355X, one GPU).
+
+Workload (one "step" = one pass of the hot path over the batch, inputs
+resident in HBM):
+  * map 4096 x 4096: log-normal field exp(0.5 g), g ~ N(0,1) (seed 1001),
+    smoothed with a circular Gaussian of sigma 3 px, scaled to mean
+    1/((1-k)^2-g^2) = 5 (k = g = 0.4)
+  * three source profiles (Gaussian, uniform disc, thin disc) at three bands
+    with half-light radii 2 / 4 / 8 px -> 9 convolved (mu, dmu/dr) maps
+  * 100 000 straight trajectories (origin and angle uniform, 200 px over the
+    100-day baseline), each with the three profiles -> 300 000 light curves
+  * 100 irregular epochs per band (300 per curve) against one synthetic
+    observed curve (0.05 mag Gaussian noise)
+  * per step: map spectrum, 9 fused convolutions, chi2 + 7-parameter adjoint
+    of every curve, stable sort of the chi2 values and their 100-bin table
+The metric counts light curves (1 trajectory x 1 profile, all bands and
+epochs) per second.  The same step is also timed on the FP64 plain path
+(reference-structured: full-size kernels, separate multiply, synchronised
+stages, two-launch chi2), and the FP64 CPU oracle is timed on a bounded
+sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 4096
+NTRAJ = 100_000
+EPOCHS = 100
+TSPAN = 100.0
+SHAPES = ("gaussian", "uniform", "exponential")
+BAND_SCALE = (1.0, 2.0, 4.0)
+R0 = 2.0
+NBINS = 100
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def make_map(seed=1001, n=N, sigma=3.0, mean=5.0):
+    rng = np.random.default_rng(seed)
+    f = np.exp(0.5 * rng.standard_normal((n, n)))
+    k = np.fft.fftfreq(n)
+    kr = np.fft.rfftfreq(n)
+    H = np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * (k[:, None] ** 2 + kr[None, :] ** 2))
+    f = np.fft.irfft2(np.fft.rfft2(f) * H, s=(n, n))
+    return f * (mean / f.mean())
+
+
+def make_inputs(seed=2001):
+    rng = np.random.default_rng(seed)
+    L = 200.0
+    v = L / TSPAN
+    margin = L + 16.0
+    x0 = rng.uniform(margin, N - 1 - margin, NTRAJ)
+    y0 = rng.uniform(margin, N - 1 - margin, NTRAJ)
+    th = rng.uniform(0.0, 2.0 * np.pi, NTRAJ)
+    dm = rng.normal(0.0, 0.05, NTRAJ)
+    traj = np.stack([x0, y0, th, np.full(NTRAJ, v), np.full(NTRAJ, R0), np.zeros(NTRAJ), dm], 1)
+    params = np.repeat(traj, len(SHAPES), axis=0)  # curve = (trajectory, profile)
+    prof = np.tile(np.arange(len(SHAPES), dtype=np.int32), NTRAJ)
+    times = [np.sort(rng.uniform(0.0, TSPAN, EPOCHS)) for _ in BAND_SCALE]
+    truth = np.array([N / 2 + 3.3, N / 2 - 7.1, 0.7, v, R0, 0.0, 0.0])
+    return params, prof, times, truth, rng
+
+
+def bilinear(img, x, y):
+    ix = np.minimum(np.floor(x).astype(int), img.shape[1] - 2)
+    iy = np.minimum(np.floor(y).astype(int), img.shape[0] - 2)
+    fx, fy = x - ix, y - iy
+    return ((1 - fy) * ((1 - fx) * img[iy, ix] + fx * img[iy, ix + 1])
+            + fy * ((1 - fx) * img[iy + 1, ix] + fx * img[iy + 1, ix + 1]))
+
+
+def algorithmic_bytes(name, esz, ncurves, nsamp):
+    """HBM bytes one launch must move at least (value bytes of its operands)."""
+    W = N // 2 + 1
+    spec = N * W * 2 * esz          # one half spectrum
+    real = N * N * esz              # one real map
+    return {
+        "conv_node_col": spec + 2 * spec,              # read map spectrum, write 2 spectra
+        "fft_row_c2r_node": 2 * spec + 2 * real,       # read 2 spectra, write (mu, dmu/dr)
+        "fft_row_r2c": N * N * 8 + spec,               # read FP64 map, write spectrum
+        "fft_col_fwd": 2 * spec,
+        "lc_chi2_grad": ncurves * (7 * 8 + 8 + 8 * 8) + nsamp * 4 * 2 * esz,
+    }.get(name)
+
+
+def run_step(ctx, ws, plain=False):
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    ws["spec"].update(ws["map"])
+    ws["nodes"].compute(plain)
+    chi2, grad = lcm.chi2_grad(ctx, ws["nodes"], ws["obs"], ws["params"], ws["prof"], ws["obs_idx"],
+                               plain=plain)
+    perm, _, sk = api.sort_f64(ctx, chi2, sorted_keys=True)
+    lo, npb = api.bin_edges(ctx, sk, ws["bins"])
+    sb = ws["sb"]
+    sb.lower_ind[:], sb.n_per_bin[:] = lo, npb
+    be = api.bin_chi2_GPU(ctx, sb, sk)
+    return chi2, grad, lo, npb, be
+
+
+def setup(ctx, prec, m_dev, params, prof, times, mobs, sig):
+    import torch
+
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    spec = api.MapSpectrum(ctx, m_dev, prec)
+    rr = [[[R0 * sc] for sc in BAND_SCALE] for _ in SHAPES]
+    nodes = lcm.NodeSet(ctx, spec, SHAPES, rr, BAND_SCALE, compute=False)
+    obs = lcm.Observations(ctx, len(BAND_SCALE), [[(times[b], mobs[b], sig[b]) for b in range(len(BAND_SCALE))]])
+    sb = api.Chi2SortBins(ctx, 1, NBINS)
+    bins = api.mpd_bins(NBINS, 1.0, 1.0e7, True)
+    return dict(spec=spec, nodes=nodes, obs=obs, map=m_dev, sb=sb, bins=bins,
+                params=torch.tensor(params, device="cuda"), prof=torch.tensor(prof, device="cuda"),
+                obs_idx=torch.zeros(len(prof), dtype=torch.int32, device="cuda"))
+
+
+def time_steps(ctx, ws, steps, warmup, plain, timing):
+    import torch
+
+    for _ in range(warmup):
+        run_step(ctx, ws, plain)
+    torch.cuda.synchronize()
+    if timing:
+        ctx.timing_reset()
+        ctx.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = run_step(ctx, ws, plain)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    kern = {}
+    if timing:
+        ctx.timing(False)
+        kern = ctx.timing_all()
+    return dt, out, kern
+
+
+def cpu_baseline(m, params, prof, times, mobs, sig, nodes64_host, budget_s=20.0):
+    """FP64 oracle (C++, OpenMP) on a bounded sample of the same workload:
+    one 4096^2 FFT convolution (3 2-D FFTs) and a slice of the curves; the
+    step time is extrapolated to 37 2-D FFTs (map once + 9 nodes x value and
+    derivative x kernel transform and inverse) plus all 300 000 curves."""
+    from oracle import oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    prof0 = orc.profile("gaussian", R0)
+    ker = orc.set_kernel(prof0, N, N)
+    t0 = time.perf_counter()
+    orc.fft_convolve(m, ker)
+    t_conv = time.perf_counter() - t0
+    t_fft = t_conv / 3.0
+    # curves of profile 0 (Gaussian): node maps of its 3 bands
+    sel = np.nonzero(prof == 0)[0]
+    ns = 30000
+    off = np.cumsum([0] + [len(t) for t in times]).astype(np.int32)
+    tt, mo, sg = np.concatenate(times), np.concatenate(mobs), np.concatenate(sig)
+    t0 = time.perf_counter()
+    orc.lc_chi2_grad(nodes64_host, 3, np.array([0, 1, 2], np.int32), np.ones(3, np.int32),
+                     np.array([R0 * s for s in BAND_SCALE]), np.arange(3, dtype=np.int32),
+                     np.array(BAND_SCALE), params[sel[:ns]], np.zeros(ns, np.int32), np.zeros(ns, np.int32),
+                     off, tt, mo, sg)
+    t_curves = (time.perf_counter() - t0) / ns
+    keys = np.random.default_rng(0).lognormal(8, 2, 300_000)
+    t0 = time.perf_counter()
+    orc.stable_argsort(keys)
+    t_sort = time.perf_counter() - t0
+    ncurves = len(prof)
+    t_step = 37 * t_fft + ncurves * t_curves + t_sort
+    return {"value": ncurves / t_step, "unit": "light curves/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 oracle on the box's host: one 4096^2 FFT convolution ({t_conv:.2f} s, "
+                       f"3 2-D FFTs), {ns} curves x 300 epochs ({t_curves * ns:.2f} s) and a "
+                       f"300k-key stable sort ({t_sort:.2f} s), extrapolated to one step = 37 2-D "
+                       f"FFTs + {ncurves} curves + sort ({t_step:.1f} s)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    t_setup = time.perf_counter()
+    m = make_map()
+    params, prof, times, truth, rng = make_inputs()
+    ctx = api.Context(0)
+    m_dev = torch.tensor(m, device="cuda")
+
+    # synthetic observed curve: the true trajectory on the FP64 Gaussian maps + noise
+    s64 = api.MapSpectrum(ctx, m_dev, "f64")
+    mobs, sig = [], []
+    for b, sc in enumerate(BAND_SCALE):
+        v = s64.convolve("gaussian", R0 * sc).cpu().numpy()
+        t = times[b]
+        x = truth[0] + truth[3] * np.cos(truth[2]) * t
+        y = truth[1] + truth[3] * np.sin(truth[2]) * t
+        mobs.append(-2.5 * np.log10(bilinear(v, x, y)) + rng.normal(0.0, 0.05, t.size))
+        sig.append(np.full(t.size, 0.05))
+    s64.close()
+    ncurves, nsamp = len(prof), len(prof) * EPOCHS * len(BAND_SCALE)
+
+    # ---- FP32 fused path (the metric)
+    ws32 = setup(ctx, "f32", m_dev, params, prof, times, mobs, sig)
+    t_setup = time.perf_counter() - t_setup
+    dt32, out32, kern = time_steps(ctx, ws32, args.steps, args.warmup, plain=False, timing=True)
+    chi32, grad32, lo32, npb32, be32 = out32
+
+    # ---- FP64 plain path (same workload)
+    plain = None
+    err = {}
+    if not args.no_plain:
+        ws64 = setup(ctx, "f64", m_dev, params, prof, times, mobs, sig)
+        k64 = max(2, args.steps // 5)
+        dt64, out64, _ = time_steps(ctx, ws64, k64, 1, plain=True, timing=False)
+        chi64, grad64, lo64, npb64, be64 = out64
+        fin = torch.isfinite(chi64)
+        rel = ((chi32 - chi64).abs() / chi64.abs())[fin]
+        gs = grad64.abs().amax(dim=0)
+        n32 = ws32["nodes"].nodes[..., 0].double()
+        n64 = ws64["nodes"].nodes[..., 0]
+        map_err = float(((n32 - n64).abs() / n64.abs()).max())
+        err = {"chi2_max_rel_err": float(rel.max()), "chi2_median_rel_err": float(rel.median()),
+               "grad_max_err_over_scale": float(((grad32 - grad64).abs() / gs).max()),
+               "convolved_map_max_rel_err": map_err,
+               "lightcurve_max_err_mag": 2.5 / np.log(10) * map_err,
+               "bins_identical_to_fp64": bool(np.array_equal(lo32, lo64) and np.array_equal(npb32, npb64)),
+               "curves_changing_bin": int(np.abs(npb32.astype(np.int64) - npb64.astype(np.int64)).sum() // 2)}
+        plain = {"value": ncurves / dt64, "ms_per_step": dt64 * 1e3, "steps": k64, "dtype": "f64"}
+        del ws64
+        torch.cuda.empty_cache()
+
+    # ---- roofline of the dominant kernel
+    dom = max((k for k in kern if algorithmic_bytes(k, 4, ncurves, nsamp)), key=lambda k: kern[k][0])
+    ms_tot, launches = kern[dom]
+    avg_s = ms_tot / launches / 1e3
+    ab = algorithmic_bytes(dom, 4, ncurves, nsamp)
+    roof = {"kernel": dom, "bound": "hbm", "achieved": ab / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": None,
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": avg_s * 1e3}
+
+    cpu = None
+    if not args.no_cpu:
+        nodes64 = np.empty((3, N, N, 2))
+        s64 = api.MapSpectrum(ctx, m_dev, "f64")
+        for b, sc in enumerate(BAND_SCALE):
+            nodes64[b] = s64.convolve("gaussian", R0 * sc, value=False, node=True).cpu().numpy()
+        s64.close()
+        cpu = cpu_baseline(m, params, prof, times, mobs, sig, nodes64)
+
+    line = {
+        "metric": "light curves/s with chi2+grad @4096² map, FP32 vs FP64 plain, ≤1e-5 chi2 err",
+        "value": ncurves / dt32, "unit": "light curves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt32 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded log-normal map, random trajectories)",
+        "config": {"workload": "4096^2 map, 3 profiles x 3 bands (2/4/8 px), 1e5 trajectories x 3 profiles, "
+                               "100 epochs/band, chi2 + 7-param adjoint + sort + 100-bin table",
+                   "map": N, "curves": ncurves, "epochs_per_curve": EPOCHS * len(BAND_SCALE),
+                   "nodes": len(SHAPES) * len(BAND_SCALE)},
+        "fp64_plain": plain, "speedup_vs_fp64_plain": (ncurves / dt32) / plain["value"] if plain else None,
+        "precision": err, "roofline": roof, "cpu_baseline": cpu,
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kern.items())},
+        "setup_s": t_setup,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

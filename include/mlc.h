@@ -110,7 +110,9 @@ int mlc_profile_size(int shape, double rhalf, double pix, double rin, double a);
  * device from the n x n profile, the pointwise product is taken inside the
  * column FFT and nothing of size Nx*Ny is staged from the host.
  *   d_value: Ny x Nx map in the handle's precision (float or double), or NULL
- *   d_node : Ny x Nx interleaved (value, d value / d rhalf) pairs, or NULL
+ *   d_node : Ny x Nx interleaved (value, d value / d rhalf) pairs, or NULL;
+ *            stored in 4 x 4 pixel tiles: pixel (x, y) is pair number
+ *            ((y/4)*(Nx/4) + x/4)*16 + (y%4)*4 + x%4 (needs Nx, Ny % 4 == 0)
  * At least one of them must be given. */
 int mlc_map_convolve(mlc_map* map, int shape, double rhalf, double pix,
                      double rin, double a, void* d_value, void* d_node);
@@ -216,7 +218,8 @@ int mlc_chi2_multiband(mlc_ctx* ctx, const float* d_f_obs, const float* d_df_obs
  * chi2 of kernel_chi2 700-750 to magnitudes with offsets).
  *
  *   d_nodes : nnodes consecutive Ny x Nx maps of interleaved (mu, dmu/drad)
- *             in `precision` (as written by mlc_map_convolve d_node)
+ *             in `precision`, 4 x 4 pixel-tiled as written by
+ *             mlc_map_convolve d_node (Nx, Ny >= 4 and divisible by 4)
  *   node_off/node_cnt (host, P*B): nodes of profile p, band b are
  *             node_off[p*B+b] .. +node_cnt-1, radii ascending node_rad[]
  *   band_scale (host, B): radius of band b at time t = band_scale[b]*(r0+w*t)

@@ -124,13 +124,19 @@ class MapSpectrum:
         _dev(self.ctx, dmap, torch.float64, "map")
         self.ctx.check(lib().mlc_map_update(self.h, ptr(dmap)))
 
-    def convolve(self, shape, rhalf, pix=1.0, rin=0.0, a=1.0, value=True, node=False, plain=False):
-        """Convolved map (Ny, Nx) and/or node map (Ny, Nx, 2) = (mu, dmu/drhalf)."""
+    def convolve(self, shape, rhalf, pix=1.0, rin=0.0, a=1.0, value=True, node=False, plain=False,
+                 tiled=False):
+        """Convolved map (Ny, Nx) and/or node map (Ny, Nx, 2) = (mu, dmu/drhalf).
+
+        The library writes node maps in 4 x 4 pixel tiles; they are returned
+        row-major unless tiled=True."""
         v = torch.empty((self.Ny, self.Nx), dtype=self.dtype, device=self.ctx.device) if value else None
         nd = torch.empty((self.Ny, self.Nx, 2), dtype=self.dtype, device=self.ctx.device) if node else None
         f = lib().mlc_map_convolve_plain if plain else lib().mlc_map_convolve
         self.ctx.check(f(self.h, _shape(shape), float(rhalf), float(pix), float(rin), float(a),
                          ptr(v), ptr(nd)))
+        if node and not tiled:
+            nd = untile(nd)
         if value and node:
             return v, nd
         return v if value else nd
@@ -145,6 +151,15 @@ class MapSpectrum:
             self.close()
         except Exception:
             pass
+
+
+def untile(t):
+    """Row-major (..., Ny, Nx, 2) copy of a node map stored in 4 x 4 pixel tiles."""
+    *lead, Ny, Nx, two = t.shape
+    v = t.reshape(*lead, Ny // 4, Nx // 4, 4, 4, two)
+    nl = len(lead)
+    perm = list(range(nl)) + [nl, nl + 2, nl + 1, nl + 3, nl + 4]
+    return v.permute(*perm).reshape(*lead, Ny, Nx, two)
 
 
 def gerlumph_normalise(ctx, counts, avgmu, avgN, k, g):

@@ -1,0 +1,56 @@
+// mlc_fft.hpp -- the map handle and the precision-templated FFT entry points
+// shared by the C ABI (mlc_fft.hip) and the kernel translation units
+// (mlc_fft_f32.hip, mlc_fft_f64.hip).
+#pragma once
+
+#include "mlc_fft_core.hpp"
+#include "mlc_internal.hpp"
+#include "mlc_profile.hpp"
+
+struct mlc_map {
+  mlc_ctx* ctx = nullptr;
+  int precision = MLC_F32;
+  int Nx = 0, Ny = 0, W = 0;
+  int subtract_mean = 0;
+  void* fmap = nullptr;    // Ny x W complex (precision), spectrum / (Nx*Ny)
+  double* mean = nullptr;  // device FP64: map mean (0 when not subtracted)
+  void* g0 = nullptr;      // Ny x W complex scratch (value column output)
+  void* g1 = nullptr;      // Ny x W complex scratch (derivative column output)
+};
+
+namespace mlc {
+
+// Throws unless Nx is even, Nx/2 and Ny factor over {2,3,5} and the offsets
+// fit the kernels' 32-bit indexing.
+void fft_check_dims(int Nx, int Ny);
+
+// n x n profile and d/drhalf, normalised, into device FP64 arrays.
+void fft_profile(mlc_ctx* ctx, const ProfileParams& P, double* prof, double* dprof);
+
+// Map spectrum of d_map into m->fmap (and its FP64 mean into m->mean).
+template <class T>
+void fft_map_transform(mlc_map* m, const double* d_map);
+// Fused convolution (see mlc_map_convolve).
+template <class T>
+void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
+// Reference-structured convolution (see mlc_map_convolve_plain).
+template <class T>
+void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
+// Unnormalised 2-D real transforms; fft_c2r_2d overwrites `work`.
+template <class T>
+void fft_r2c_2d(mlc_ctx* ctx, const T* in, cpx<T>* out, int Nx, int Ny);
+template <class T>
+void fft_c2r_2d(mlc_ctx* ctx, cpx<T>* work, T* out, int Nx, int Ny);
+
+extern template void fft_map_transform<float>(mlc_map*, const double*);
+extern template void fft_map_transform<double>(mlc_map*, const double*);
+extern template void fft_convolve_fused<float>(mlc_map*, const ProfileParams&, void*, void*);
+extern template void fft_convolve_fused<double>(mlc_map*, const ProfileParams&, void*, void*);
+extern template void fft_convolve_plain<float>(mlc_map*, const ProfileParams&, void*, void*);
+extern template void fft_convolve_plain<double>(mlc_map*, const ProfileParams&, void*, void*);
+extern template void fft_r2c_2d<float>(mlc_ctx*, const float*, cpx<float>*, int, int);
+extern template void fft_r2c_2d<double>(mlc_ctx*, const double*, cpx<double>*, int, int);
+extern template void fft_c2r_2d<float>(mlc_ctx*, cpx<float>*, float*, int, int);
+extern template void fft_c2r_2d<double>(mlc_ctx*, cpx<double>*, double*, int, int);
+
+}  // namespace mlc

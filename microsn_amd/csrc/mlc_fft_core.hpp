@@ -128,6 +128,41 @@ MLC_HD void dft5(cpx<T>* v) {
   v[3] = sub(m2, n2);
 }
 
+template <class T, bool INV>
+MLC_HD void dft16(cpx<T>* v) {
+  // 16 = 4 x 4: DFT4 over n2 for each n1 (x[n1 + 4 n2]), twiddle W_16^(n1 k2),
+  // DFT4 over n1 for each k2; output X[k2 + 4 k1].
+  const T c1 = (T)0.92387953251128675612818318939678828682;  // cos(pi/8)
+  const T s1 = (T)0.38268343236508977172845998403039886676;  // sin(pi/8)
+  const T h = (T)0.70710678118654752440084436210484903928;
+  cpx<T> y[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    cpx<T> a[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    dft4<T, INV>(a);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) y[n1][k2] = a[k2];
+  }
+  // W_16^m = cos(2 pi m/16) -+ i sin(2 pi m/16), m = n1*k2 in {1,2,3,4,6,9}
+  const T cs[10] = {1, c1, h, s1, 0, 0, -h, 0, 0, -c1};
+  const T sn[10] = {0, s1, h, c1, 1, 0, h, 0, 0, -s1};
+#pragma unroll
+  for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+    for (int k2 = 1; k2 < 4; ++k2) {
+      const T c = cs[n1 * k2], sv = INV ? -sn[n1 * k2] : sn[n1 * k2];
+      const cpx<T> a = y[n1][k2];
+      y[n1][k2] = mk(a.x * c + a.y * sv, a.y * c - a.x * sv);  // a * (c - i sv)
+    }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    cpx<T> a[4] = {y[0][k2], y[1][k2], y[2][k2], y[3][k2]};
+    dft4<T, INV>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = a[k1];
+  }
+}
+
 template <class T, int R, bool INV>
 MLC_HD void dft(cpx<T>* v) {
   if constexpr (R == 2) dft2<T, INV>(v);
@@ -135,32 +170,64 @@ MLC_HD void dft(cpx<T>* v) {
   else if constexpr (R == 4) dft4<T, INV>(v);
   else if constexpr (R == 5) dft5<T, INV>(v);
   else if constexpr (R == 8) dft8<T, INV>(v);
+  else if constexpr (R == 16) dft16<T, INV>(v);
 }
 
 // ------------------------------------------------------------------ stage
-// Loads, twiddles and transforms butterfly j into v[0..R).  tw is the table
-// W_L^m = exp(-2 pi i m/L), m in [0, L) (conjugated on use for INV).
-template <class T, int R, bool INV>
-MLC_HD void stage_load(const cpx<T>* x, int L, int Ns, int j, const cpx<T>* tw, cpx<T>* v) {
+// Index map of the sequence in memory (identity; the LDS kernels use it).
+struct NoPad {
+  MLC_HD int operator()(int i) const { return i; }
+};
+// Twiddle source W_L^m = exp(-2 pi i m/L), m in [0, L): a full table.
+template <class T>
+struct TableTw {
+  const cpx<T>* tw;
+  MLC_HD cpx<T> operator()(int m) const { return tw[m]; }
+};
+// The same from a quarter-wave table q[r] = W_L^r, r in [0, L/4] (L % 4 == 0):
+// W_L^(a L/4 + r) = (-i)^a W_L^r.
+template <class T>
+struct QuarterTw {
+  const cpx<T>* q;
+  int QL;  // L/4
+  MLC_HD cpx<T> operator()(int m) const {
+    const int a = (m >= QL) + (m >= 2 * QL) + (m >= 3 * QL);
+    const cpx<T> w = q[m - a * QL];
+    switch (a) {
+      case 0: return w;
+      case 1: return mk(w.y, -w.x);
+      case 2: return mk(-w.x, -w.y);
+      default: return mk(-w.y, w.x);
+    }
+  }
+};
+
+// Loads, twiddles and transforms butterfly j into v[0..R) (conjugate
+// twiddles for INV).
+template <class T, int R, bool INV, class TW, class PAD = NoPad>
+MLC_HD void stage_load(const cpx<T>* x, int L, int Ns, int j, const TW& tw, cpx<T>* v,
+                       PAD pad = PAD()) {
   const int stride = L / R;
   const int k = j % Ns;
   const int tstep = k * (L / (Ns * R));
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = x[j + r * stride];
+  for (int r = 0; r < R; ++r) v[r] = x[pad(j + r * stride)];
+  if (k != 0) {
 #pragma unroll
-  for (int r = 1; r < R; ++r) {
-    cpx<T> w = tw[r * tstep];
-    v[r] = INV ? mulc(v[r], w) : mul(v[r], w);
+    for (int r = 1; r < R; ++r) {
+      cpx<T> w = tw(r * tstep);
+      v[r] = INV ? mulc(v[r], w) : mul(v[r], w);
+    }
   }
   dft<T, R, INV>(v);
 }
 
-template <class T, int R>
-MLC_HD void stage_store(cpx<T>* y, int Ns, int j, const cpx<T>* v) {
+template <class T, int R, class PAD = NoPad>
+MLC_HD void stage_store(cpx<T>* y, int Ns, int j, const cpx<T>* v, PAD pad = PAD()) {
   const int k = j % Ns;
   const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-  for (int q = 0; q < R; ++q) y[base + q * Ns] = v[q];
+  for (int q = 0; q < R; ++q) y[pad(base + q * Ns)] = v[q];
 }
 
 // ------------------------------------------------------------------ real FFT
@@ -195,17 +262,19 @@ MLC_HD void c2r_pre(cpx<T> Xk, cpx<T> Xm, cpx<T> w, cpx<T>* Zk, cpx<T>* Zm) {
   *Zm = conj(sub(S, iD));
 }
 
-// Radix plan: factors of L, largest first among {8, 4, 2} then 5, 3.
+// Radix plan: factors of L, largest first among {16, 8, 4, 2} (none above
+// max_radix) then 5, 3.
 struct Plan {
   int L = 0;
   int n = 0;
   int r[24] = {0};
 };
 
-inline bool make_plan(int L, Plan* p) {
+inline bool make_plan(int L, Plan* p, int max_radix = 16) {
   p->L = L;
   p->n = 0;
   int m = L;
+  while (max_radix >= 16 && m % 16 == 0) { p->r[p->n++] = 16; m /= 16; }
   while (m % 8 == 0) { p->r[p->n++] = 8; m /= 8; }
   while (m % 4 == 0) { p->r[p->n++] = 4; m /= 4; }
   while (m % 2 == 0) { p->r[p->n++] = 2; m /= 2; }

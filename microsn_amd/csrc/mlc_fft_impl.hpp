@@ -1,0 +1,755 @@
+// mlc_fft_impl.hpp -- the library's own 2-D FFT and the fused map x profile
+// convolution (replaces cuFFT D2Z/Z2D + kernel_multiplyFFTs +
+// Kernel::setKernel, gpu_functions.cu:151-200, 522-577;
+// magnification_map.cpp:172-194).
+//
+// Layout.  A real Ny x Nx map is row-major; its half spectrum is Ny rows of
+// W = Nx/2+1 complex values (the cuFFT D2Z layout).  Every transform runs
+// inside LDS:
+//   row pass    one workgroup per row: the real row of length Nx is folded
+//               into a complex sequence of length M = Nx/2 (even/odd
+//               samples), transformed, and unfolded (mlc_fft_core.hpp);
+//   column pass one workgroup per CB adjacent columns of the half spectrum,
+//               each column transformed by 1024/CB threads; global reads
+//               and writes move CB*8 (or CB*16) contiguous bytes per row and
+//               workgroups are numbered so that neighbouring column groups
+//               run on the same XCD (shared L2).
+// Per source profile (mlc_map_convolve) nothing of size Nx*Ny is staged from
+// the host: the n x n profile is evaluated on the device, its wrapped kernel
+// is transformed along x directly (n nonzero rows, n nonzero columns), and a
+// single column kernel does forward FFT of the kernel column -> multiply by
+// the map spectrum column -> inverse FFT, for the value and the d/drhalf
+// kernel.  A row pass then writes the real maps.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "mlc_fft_core.hpp"
+#include "mlc_internal.hpp"
+#include "mlc_fft.hpp"
+#include "mlc_profile.hpp"
+#include "mlc_tiles.hpp"
+
+using namespace mlc;
+
+namespace {
+
+constexpr int kColThreads = 1024;
+constexpr size_t kLdsBytes = 160 * 1024;
+
+// ------------------------------------------------------------------ device FFT
+// Twiddles live in LDS as W_L^m, m < QL: a quarter wave (QL = L/4) when
+// 4 | L, else the full table (QL = L); QuarterTw reconstructs the rest.
+__host__ __device__ inline int tw_len(int L) { return (L % 4 == 0) ? L / 4 : L; }
+
+template <class T, int R, int G, int EPT, bool INV, class TW, class PAD>
+__device__ __forceinline__ void dev_stage(cpx<T>* buf, int L, int Ns, const TW& tw, PAD pad, int g) {
+  constexpr int BMAX = (EPT + R - 1) / R;
+  cpx<T> v[BMAX][R];
+  const int nb = L / R;
+#pragma unroll
+  for (int i = 0; i < BMAX; ++i) {
+    int j = g + i * G;
+    if (j < nb) stage_load<T, R, INV>(buf, L, Ns, j, tw, v[i], pad);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < BMAX; ++i) {
+    int j = g + i * G;
+    if (j < nb) stage_store<T, R>(buf, Ns, j, v[i], pad);
+  }
+  __syncthreads();
+}
+
+// All stages of `plan` on buf by the G threads of one group (g = index in
+// the group).  Every thread of the workgroup must call this: each stage
+// synchronises the whole workgroup, and all groups run the same plan.
+template <class T, int G, int EPT, bool INV, class TW, class PAD>
+__device__ void dev_fft(cpx<T>* buf, const Plan& plan, const TW& tw, PAD pad, int g) {
+  int Ns = 1;
+  for (int s = 0; s < plan.n; ++s) {
+    const int R = plan.r[s];
+    // radix 16 only where the register budget allows it (plans for
+    // 1024-thread column kernels never contain it)
+    if (G <= 256 && R == 16) dev_stage<T, 16, G, EPT, INV>(buf, plan.L, Ns, tw, pad, g);
+    else if (R == 8) dev_stage<T, 8, G, EPT, INV>(buf, plan.L, Ns, tw, pad, g);
+    else if (R == 4) dev_stage<T, 4, G, EPT, INV>(buf, plan.L, Ns, tw, pad, g);
+    else if (R == 2) dev_stage<T, 2, G, EPT, INV>(buf, plan.L, Ns, tw, pad, g);
+    else if (R == 5) dev_stage<T, 5, G, EPT, INV>(buf, plan.L, Ns, tw, pad, g);
+    else dev_stage<T, 3, G, EPT, INV>(buf, plan.L, Ns, tw, pad, g);
+    Ns *= R;
+  }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this numbering gives
+// each XCD a contiguous range of logical blocks.  The grid is launched with
+// 8*ceil(nb/8) workgroups; ids >= nb return at once.
+__device__ __forceinline__ int xcd_block(int nb) {
+  const int per = (nb + 7) >> 3;
+  const int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
+}
+
+__device__ __forceinline__ int wrap_row(int y, int h, int N) {
+  int s = y + h;
+  return s >= N ? s - N : s;
+}
+
+// ------------------------------------------------------------------ row kernels
+// LDS of a row kernel: [twiddles W_M^m, m < tw_len(M)] [row buffer(s)].
+// Forward real FFT of one row per workgroup.  in: Ny x Nx of TIN; out: Ny x W.
+// With shift != NULL the FP64 value *shift is subtracted before rounding to T.
+template <class T, class TIN, int G>
+__global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T>* __restrict__ out,
+                                               int Nx, const double* __restrict__ shift, Plan plan,
+                                               const cpx<T>* __restrict__ twM,
+                                               const cpx<T>* __restrict__ twN) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int y = blockIdx.x, g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
+  cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* buf = q + QL;
+  const NoPad P;
+  for (int i = g; i < QL; i += G) q[i] = twM[i];
+  const cpx<TIN>* row = reinterpret_cast<const cpx<TIN>*>(in + (size_t)y * Nx);
+  const double sh = shift ? *shift : 0.0;
+  for (int n = g; n < M; n += G) {
+    cpx<TIN> p = row[n];
+    buf[P(n)] = mk<T>((T)((double)p.x - sh), (T)((double)p.y - sh));
+  }
+  __syncthreads();
+  dev_fft<T, G, 16, false>(buf, plan, QuarterTw<T>{q, QL}, P, g);
+  cpx<T>* o = out + (size_t)y * W;
+  for (int k = g; k <= (M >> 1); k += G) {
+    cpx<T> Xk, Xm;
+    r2c_post(buf[P(k == 0 ? 0 : k)], buf[P(k == 0 ? 0 : M - k)], twN[k], &Xk, &Xm);
+    o[k] = Xk;
+    if (M - k != k) o[M - k] = Xm;
+  }
+}
+
+// Loads the folded inverse input of one row into buf (see c2r_pre).
+template <class T, int G>
+__device__ __forceinline__ void c2r_fill(cpx<T>* buf, const cpx<T>* __restrict__ X, int M,
+                                         const cpx<T>* __restrict__ twN, int g) {
+  const NoPad P;
+  for (int k = g; k <= (M >> 1); k += G) {
+    cpx<T> Zk, Zm;
+    c2r_pre(X[k], X[M - k], twN[k], &Zk, &Zm);
+    buf[P(k)] = Zk;
+    if (k > 0 && M - k != k) buf[P(M - k)] = Zm;
+  }
+}
+
+// Inverse real FFT of one row per workgroup (unnormalised).
+// MODE 0: out is T[Ny][Nx] = x;  MODE 1: out is T[Ny][Nx] = x + *shift;
+// MODE 2: out is the node map (x0 + *shift, x1) of the spectra in0, in1 in
+// 4 x 4 pixel tiles (mlc_tiles.hpp); rows are numbered XCD-contiguously so
+// the four rows of a tile are written through one L2.
+template <class T, int G, int MODE>
+__global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
+                                               const cpx<T>* __restrict__ in1, T* __restrict__ out,
+                                               int Nx, int Ny, const double* __restrict__ shift,
+                                               Plan plan, const cpx<T>* __restrict__ twM,
+                                               const cpx<T>* __restrict__ twN) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int y = MODE == 2 ? xcd_block(Ny) : (int)blockIdx.x;
+  if (y >= Ny) return;
+  const int g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
+  cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* buf0 = q + QL;
+  const NoPad P;
+  for (int i = g; i < QL; i += G) q[i] = twM[i];
+  c2r_fill<T, G>(buf0, in0 + (size_t)y * W, M, twN, g);
+  cpx<T>* buf1 = buf0 + M;
+  if constexpr (MODE == 2) c2r_fill<T, G>(buf1, in1 + (size_t)y * W, M, twN, g);
+  __syncthreads();
+  const QuarterTw<T> tw{q, QL};
+  dev_fft<T, G, 16, true>(buf0, plan, tw, P, g);
+  const T sh = (MODE != 0 && shift) ? (T)*shift : (T)0;
+  if constexpr (MODE == 2) {
+    dev_fft<T, G, 16, true>(buf1, plan, tw, P, g);
+    // (value, deriv) pairs of pixels 2n and 2n+1: adjacent in their tile row
+    cpx<T>* o = reinterpret_cast<cpx<T>*>(out);
+    for (int n = g; n < M; n += G) {
+      cpx<T> a = buf0[P(n)], b = buf1[P(n)];
+      cpx<T>* dst = o + tile_offset(2 * n, y, Nx);
+      dst[0] = mk<T>(a.x + sh, b.x);
+      dst[1] = mk<T>(a.y + sh, b.y);
+    }
+  } else {
+    cpx<T>* o = reinterpret_cast<cpx<T>*>(out + (size_t)y * Nx);
+    for (int n = g; n < M; n += G) {
+      cpx<T> a = buf0[P(n)];
+      o[n] = mk<T>(a.x + sh, a.y + sh);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ column kernels
+// LDS of a column kernel: [twiddles W_Ny^m, m < tw_len(Ny)] [CB column buffers
+// of LS elements].  FAST == false (columns too long for that layout): the
+// twiddles are read from the global table.
+// Global <-> LDS: thread t moves column c = t % CB, rows t/CB + i*G, so a
+// wavefront touches CB contiguous elements of 64/CB rows.  Offsets fit in 32
+// bits (Ny*W < 2^30 is checked on the host).
+
+// In-place complex FFT along y of every column of an Ny x W array, CB columns
+// per workgroup, times `sc`.
+template <class T, int CB, int EPT, bool INV, bool FAST>
+__global__ __launch_bounds__(kColThreads) void k_col_fft(cpx<T>* __restrict__ data, int Ny, int W,
+                                                         T sc, Plan plan,
+                                                         const cpx<T>* __restrict__ tw, int LS) {
+  constexpr int G = kColThreads / CB;
+  using PadT = NoPad;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int QL = FAST ? tw_len(Ny) : 0;
+  cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* buf = q + QL;
+  const int ngroups = (W + CB - 1) / CB;
+  const int lb = xcd_block(ngroups);
+  if (lb >= ngroups) return;
+  const int t = threadIdx.x, u0 = lb * CB;
+  const int c = t % CB, y0 = t / CB, u = u0 + c;
+  const bool col_ok = u < W;
+  const PadT P;
+  for (int i = t; i < QL; i += kColThreads) q[i] = tw[i];
+  for (int y = y0; y < Ny; y += G) buf[c * LS + P(y)] = col_ok ? data[y * W + u] : mk<T>(0, 0);
+  __syncthreads();
+  if constexpr (FAST) dev_fft<T, G, EPT, INV>(buf + (t / G) * LS, plan, QuarterTw<T>{q, QL}, P, t % G);
+  else dev_fft<T, G, EPT, INV>(buf + (t / G) * LS, plan, TableTw<T>{tw}, P, t % G);
+  if (col_ok)
+    for (int y = y0; y < Ny; y += G) data[y * W + u] = scale(buf[c * LS + P(y)], sc);
+}
+
+// Fused per-profile column kernel.  Every profile of the reference is
+// symmetric about the pixel corner its kernel is centred on (setKernel puts
+// the centre at (-1/2, -1/2)), so the 2-D DFT of the wrapped kernel is
+//   K[v][u] = e^{i pi (u/Nx + v/Ny)} kappa[v][u],
+//   kappa[v][u] = sum_k a[k][u] cos((2k+1) pi v/Ny),   k < h = n/2,
+// with a[k][u] the x-cosine sums of the kernel rows h+k and h-1-k (built by
+// k_kernel_cos).  The Nyquist column u = Nx/2 of K vanishes exactly.  This
+// kernel therefore evaluates K on the fly (FP64 three-term recurrence over
+// the odd harmonics), multiplies by the map spectrum column and runs one
+// inverse FFT along y per column and per kernel (value a, derivative ad);
+// the Nyquist column of the output is written as zero.
+// Columns u in [0, M), M = W-1; CB per workgroup; per-row tables csy[v] =
+// (cos t, sin t, cos 2t), t = pi v/Ny, and per-column phx[u] = (cos, sin)(pi u/Nx).
+template <class T, int CB, int EPT, bool FAST>
+__global__ __launch_bounds__(kColThreads) void k_node_col(
+    const cpx<T>* __restrict__ fmap, const double* __restrict__ av, const double* __restrict__ ad,
+    int h, const double* __restrict__ csy, const double* __restrict__ phx,
+    cpx<T>* __restrict__ out_v, cpx<T>* __restrict__ out_d, int Ny, int W, Plan plan,
+    const cpx<T>* __restrict__ tw, int LS, int coef_lds) {
+  constexpr int G = kColThreads / CB;
+  constexpr int CH = 2;  // elements whose recurrences run together
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int QL = FAST ? tw_len(Ny) : 0;
+  cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* buf = q + QL;
+  double* coef = reinterpret_cast<double*>(buf + (size_t)CB * LS);  // [CB][h] when coef_lds
+  const int M = W - 1;
+  const int ngroups = (M + CB - 1) / CB;
+  const int lb = xcd_block(ngroups);
+  if (lb >= ngroups) return;
+  const int t = threadIdx.x, u0 = lb * CB;
+  const int c = t % CB, y0 = t / CB, u = u0 + c;
+  const bool col_ok = u < M;
+  for (int i = t; i < QL; i += kColThreads) q[i] = tw[i];
+  const double pc = col_ok ? phx[2 * u] : 0.0, ps = col_ok ? phx[2 * u + 1] : 0.0;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* a = pass ? ad : av;
+    cpx<T>* dst = pass ? out_d : out_v;
+    if (!dst) continue;  // uniform over the grid
+    const double* ac;    // coefficients of this thread's column
+    int astride;
+    if (coef_lds) {
+      for (int i = t; i < CB * h; i += kColThreads) {
+        const int cc = i / h, k = i % h;
+        coef[i] = (u0 + cc < M) ? a[(size_t)k * W + u0 + cc] : 0.0;
+      }
+      ac = coef + c * h;
+      astride = 1;
+    } else {
+      ac = a + (col_ok ? u : 0);
+      astride = W;
+    }
+    __syncthreads();
+    for (int y = y0; y < Ny; y += CH * G) {
+      double ct[CH], st[CH], c2[CH], Tm[CH], Tk[CH], S[CH];
+#pragma unroll
+      for (int e = 0; e < CH; ++e) {
+        const int ye = min(y + e * G, Ny - 1);
+        ct[e] = csy[3 * ye];
+        st[e] = csy[3 * ye + 1];
+        c2[e] = 2.0 * csy[3 * ye + 2];
+        Tm[e] = ct[e];
+        Tk[e] = ct[e];
+        S[e] = ac[0] * ct[e];
+      }
+      for (int k = 1; k < h; ++k) {
+        const double ak = ac[(size_t)k * astride];
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+          const double Tn = c2[e] * Tk[e] - Tm[e];
+          S[e] += ak * Tn;
+          Tm[e] = Tk[e];
+          Tk[e] = Tn;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < CH; ++e) {
+        const int ye = y + e * G;
+        if (ye < Ny) {
+          // K = e^{i(phi_u + theta_v)} kappa
+          const T kr = (T)(S[e] * (pc * ct[e] - ps * st[e]));
+          const T ki = (T)(S[e] * (ps * ct[e] + pc * st[e]));
+          buf[c * LS + ye] = col_ok ? mul(fmap[ye * W + u], mk<T>(kr, ki)) : mk<T>(0, 0);
+        }
+      }
+    }
+    __syncthreads();
+    if constexpr (FAST) dev_fft<T, G, EPT, true>(buf + (t / G) * LS, plan, QuarterTw<T>{q, QL}, NoPad(), t % G);
+    else dev_fft<T, G, EPT, true>(buf + (t / G) * LS, plan, TableTw<T>{tw}, NoPad(), t % G);
+    if (col_ok) {
+      for (int y = y0; y < Ny; y += G) dst[y * W + u] = buf[c * LS + y];
+      if (u == 0)
+        for (int y = y0; y < Ny; y += G) dst[y * W + M] = mk<T>(0, 0);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ profiles
+// One workgroup evaluates the n x n profile and its d/drhalf, normalises both
+// (K = f/S, dK = (df - K dS)/S) in FP64.
+__global__ __launch_bounds__(1024) void k_profile(ProfileParams P, double* __restrict__ prof,
+                                                  double* __restrict__ dprof) {
+  __shared__ double red[2][1024];
+  const int n = P.n, t = threadIdx.x;
+  double s = 0.0, ds = 0.0;
+  for (int idx = t; idx < n * n; idx += 1024) {
+    int i = idx / n, j = idx % n;
+    double x = profile_coord(j, n, P.pix), y = profile_coord(i, n, P.pix);
+    double f, df;
+    profile_value(P, hypot(x, y), &f, &df);
+    prof[idx] = f;
+    dprof[idx] = df;
+    s += f;
+    ds += df;
+  }
+  red[0][t] = s;
+  red[1][t] = ds;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      red[0][t] += red[0][t + w];
+      red[1][t] += red[1][t + w];
+    }
+    __syncthreads();
+  }
+  const double S = red[0][0], dS = red[1][0];
+  for (int idx = t; idx < n * n; idx += 1024) {
+    double K = prof[idx] / S;
+    dprof[idx] = (dprof[idx] - K * dS) / S;
+    prof[idx] = K;
+  }
+}
+
+// Cosine coefficients of the symmetric kernel (see k_node_col):
+//   a[k][u] = rho[h+k][u] + rho[h-1-k][u],
+//   rho[s][u] = sum_c P[s][c] cos(pi u (2(c-h)+1)/Nx),
+// from the table cosx[m] = cos(pi m/Nx), m < 2 Nx.  FP64.  grid (ceil(W/256), h).
+__global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ prof,
+                                                    const double* __restrict__ dprof, int n, int Nx,
+                                                    int W, const double* __restrict__ cosx,
+                                                    double* __restrict__ av, double* __restrict__ ad) {
+  const int u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, h = n >> 1;
+  if (u >= W) return;
+  const int s1 = h + k, s2 = h - 1 - k, twoN = 2 * Nx;
+  double sa = 0.0, sd = 0.0;
+  for (int cc = 0; cc < n; ++cc) {
+    const int m = (int)(((long long)u * (2 * (cc - h) + 1) % twoN + twoN) % twoN);
+    const double w = cosx[m];
+    sa += (prof[s1 * n + cc] + prof[s2 * n + cc]) * w;
+    sd += (dprof[s1 * n + cc] + dprof[s2 * n + cc]) * w;
+  }
+  av[(size_t)k * W + u] = sa;
+  if (ad) ad[(size_t)k * W + u] = sd;
+}
+
+// Full Ny x Nx wrapped kernel (Kernel::setKernel), for the plain path.
+template <class T>
+__global__ void k_make_kernel(const double* __restrict__ prof, int n, int Nx, int Ny,
+                              T* __restrict__ ker) {
+  const int h = n >> 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)Nx * Ny;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int y = (int)(i / Nx), x = (int)(i % Nx);
+    int s = wrap_row(y, h, Ny), c = wrap_row(x, h, Nx);
+    ker[i] = (s < n && c < n) ? (T)prof[(size_t)s * n + c] : (T)0;
+  }
+}
+
+// kernel_multiplyFFTs (gpu_functions.cu:571-577) without the hard-coded grid;
+// the 1/(Nx Ny) factor is already part of the map spectrum.
+template <class T>
+__global__ void k_cmul(const cpx<T>* __restrict__ fmap, cpx<T>* __restrict__ fk, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    fk[i] = mul(fmap[i], fk[i]);
+}
+
+// (value, deriv) of the plain path into the tiled node map.
+template <class T>
+__global__ void k_interleave(const T* __restrict__ a, const T* __restrict__ b, cpx<T>* __restrict__ o,
+                             int Nx, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    o[tile_offset((int)(i % Nx), (int)(i / Nx), Nx)] = mk<T>(a[i], b[i]);
+}
+
+// FP64 sum of a map: per-workgroup partials, then one workgroup adds them in
+// order (deterministic).  out[0] = sum / npix.
+__global__ __launch_bounds__(256) void k_sum_partial(const double* __restrict__ x, int64_t n,
+                                                     double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ part, int np,
+                                                   double inv_n, double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] * inv_n;
+}
+
+// ------------------------------------------------------------------ host side
+template <class T>
+const cpx<T>* twiddles(mlc_ctx* ctx, int L) {
+  auto key = std::make_pair(L, (int)sizeof(T));
+  auto it = ctx->twiddles.find(key);
+  if (it != ctx->twiddles.end()) return (const cpx<T>*)it->second;
+  std::vector<cpx<T>> h(L);
+  for (int m = 0; m < L; ++m) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)L;
+    h[m] = mk<T>((T)std::cos(a), (T)std::sin(a));
+  }
+  void* d = nullptr;
+  hip_check(hipMalloc(&d, sizeof(cpx<T>) * L), "hipMalloc(twiddles)");
+  ctx->twiddles[key] = d;
+  hip_check(hipMemcpy(d, h.data(), sizeof(cpx<T>) * L, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+  return (const cpx<T>*)d;
+}
+
+// FP64 host-built tables for the cosine-sum kernel spectrum (cached per
+// context): kind 1: cos(pi m/N), m < 2N;  kind 2: (cos t, sin t, cos 2t),
+// t = pi v/N, v < N;  kind 3: (cos, sin)(pi u/N), u <= N/2.
+const double* cos_table(mlc_ctx* ctx, int kind, int N) {
+  auto key = std::make_pair(N, 1000 + kind);
+  auto it = ctx->twiddles.find(key);
+  if (it != ctx->twiddles.end()) return (const double*)it->second;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  std::vector<double> h;
+  if (kind == 1) {
+    for (int m = 0; m < 2 * N; ++m) h.push_back((double)std::cos(pi * m / N));
+  } else if (kind == 2) {
+    for (int v = 0; v < N; ++v) {
+      long double t = pi * v / N;
+      h.push_back((double)std::cos(t));
+      h.push_back((double)std::sin(t));
+      h.push_back((double)std::cos(2 * t));
+    }
+  } else {
+    for (int u = 0; u <= N / 2; ++u) {
+      long double t = pi * u / N;
+      h.push_back((double)std::cos(t));
+      h.push_back((double)std::sin(t));
+    }
+  }
+  void* d = nullptr;
+  hip_check(hipMalloc(&d, sizeof(double) * h.size()), "hipMalloc(cos table)");
+  ctx->twiddles[key] = d;
+  hip_check(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice),
+            "hipMemcpy(cos table)");
+  return (const double*)d;
+}
+
+Plan plan_for(int L, const char* what, int max_radix = 16) {
+  Plan p;
+  if (L < 1 || !make_plan(L, &p, max_radix))
+    fail(MLC_ERR_UNSUPPORTED, std::string(what) + ": length " + std::to_string(L) +
+                                  " has a prime factor other than 2, 3, 5");
+  return p;
+}
+
+template <class K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    hip_check(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes),
+              "hipFuncSetAttribute(LDS)");
+}
+
+// Row launches: G threads with G*16 >= M (16 elements per thread at most).
+template <class F>
+void with_row_group(int M, const char* what, F&& f) {
+  using std::integral_constant;
+  if (M <= 64 * 16) f(integral_constant<int, 64>{});
+  else if (M <= 256 * 16) f(integral_constant<int, 256>{});
+  else if (M <= 1024 * 16) f(integral_constant<int, 1024>{});
+  else fail(MLC_ERR_UNSUPPORTED, std::string(what) + ": row too long");
+}
+
+template <class T>
+size_t row_lds(int M, int nbuf) {
+  return sizeof(cpx<T>) * ((size_t)tw_len(M) + (size_t)nbuf * M);
+}
+
+template <class T, class TIN>
+void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const double* shift) {
+  const int M = Nx / 2;
+  Plan p = plan_for(M, "row FFT", 8);
+  const cpx<T>* twM = twiddles<T>(ctx, M);
+  const cpx<T>* twN = twiddles<T>(ctx, Nx);
+  const size_t lds = row_lds<T>(M, 1);
+  if (lds > kLdsBytes) fail(MLC_ERR_UNSUPPORTED, "row FFT: row does not fit in LDS");
+  with_row_group(M, "row FFT", [&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    auto k = k_row_r2c<T, TIN, G>;
+    allow_lds(k, lds);
+    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, Nx, shift, p, twM, twN);
+  });
+}
+
+template <class T, int MODE>
+void row_c2r(mlc_ctx* ctx, const cpx<T>* in0, const cpx<T>* in1, T* out, int Nx, int Ny,
+             const double* shift) {
+  const int M = Nx / 2;
+  Plan p = plan_for(M, "row FFT", 8);
+  const cpx<T>* twM = twiddles<T>(ctx, M);
+  const cpx<T>* twN = twiddles<T>(ctx, Nx);
+  const size_t lds = row_lds<T>(M, MODE == 2 ? 2 : 1);
+  if (lds > kLdsBytes) fail(MLC_ERR_UNSUPPORTED, "row FFT: row does not fit in LDS");
+  with_row_group(M, "row FFT", [&](auto gc) {
+    constexpr int G = decltype(gc)::value;
+    auto k = k_row_c2r<T, G, MODE>;
+    allow_lds(k, lds);
+    const int grid = MODE == 2 ? ((Ny + 7) / 8) * 8 : Ny;
+    MLC_LAUNCH(ctx, MODE == 2 ? "fft_row_c2r_node" : "fft_row_c2r", k, dim3(grid), dim3(G), lds, in0,
+               in1, out, Nx, Ny, shift, p, twM, twN);
+  });
+}
+
+// Column launch geometry: CB adjacent columns per 1024-thread workgroup, at
+// most EPT elements per thread; FAST = padded buffers + LDS twiddles.
+struct ColGeom {
+  int CB, EPT, LS;
+  bool fast;
+  size_t lds;
+};
+
+// Geometries with a kernel instantiation, in order of preference (widest CB
+// first, then fewest elements per thread); the unpadded, global-twiddle
+// layout is the fallback for columns too long for the fast one.
+constexpr int kColCases[][3] = {{16, 8, 1}, {8, 8, 1}, {8, 16, 1}, {4, 8, 1}, {4, 16, 1},
+                                {2, 8, 1},  {2, 16, 1}, {2, 24, 1}, {1, 8, 1}, {1, 16, 1},
+                                {1, 24, 1}, {1, 16, 0}, {1, 24, 0}};
+
+template <class T>
+ColGeom col_geom(int Ny) {
+  for (auto& cs : kColCases) {
+    const int CB = cs[0], EPT = cs[1];
+    const bool fast = cs[2] != 0;
+    const int LS = Ny + 1;  // odd stride: neighbouring columns start on different banks
+    const size_t lds = sizeof(cpx<T>) * ((size_t)LS * CB + (fast ? tw_len(Ny) : 0));
+    if (lds <= kLdsBytes && Ny * CB <= kColThreads * EPT) return {CB, EPT, LS, fast, lds};
+  }
+  fail(MLC_ERR_UNSUPPORTED, "column FFT: column of " + std::to_string(Ny) + " does not fit in LDS");
+}
+
+template <class F>
+void with_col(const ColGeom& g, F&& f) {
+  using std::integral_constant;
+#define MLC_COL_CASE(cb, ept, fst)                                                         \
+  if (g.CB == cb && g.EPT == ept && g.fast == (fst != 0)) {                                \
+    f(integral_constant<int, cb>{}, integral_constant<int, ept>{}, integral_constant<bool, fst != 0>{}); \
+    return;                                                                                \
+  }
+  MLC_COL_CASE(16, 8, 1) MLC_COL_CASE(8, 8, 1) MLC_COL_CASE(8, 16, 1) MLC_COL_CASE(4, 8, 1)
+  MLC_COL_CASE(4, 16, 1) MLC_COL_CASE(2, 8, 1) MLC_COL_CASE(2, 16, 1) MLC_COL_CASE(2, 24, 1)
+  MLC_COL_CASE(1, 8, 1) MLC_COL_CASE(1, 16, 1) MLC_COL_CASE(1, 24, 1) MLC_COL_CASE(1, 16, 0)
+  MLC_COL_CASE(1, 24, 0)
+#undef MLC_COL_CASE
+  fail(MLC_ERR_UNSUPPORTED, "column FFT: no kernel for this geometry");
+}
+
+template <class T, bool INV>
+void col_fft(mlc_ctx* ctx, cpx<T>* data, int Ny, int W, T scale) {
+  Plan p = plan_for(Ny, "column FFT", 8);
+  const cpx<T>* tw = twiddles<T>(ctx, Ny);
+  ColGeom g = col_geom<T>(Ny);
+  with_col(g, [&](auto cbc, auto eptc, auto fc) {
+    constexpr int CB = decltype(cbc)::value, EPT = decltype(eptc)::value;
+    constexpr bool FAST = decltype(fc)::value;
+    int ngroups = (W + CB - 1) / CB;
+    auto k = k_col_fft<T, CB, EPT, INV, FAST>;
+    allow_lds(k, g.lds);
+    MLC_LAUNCH(ctx, INV ? "fft_col_inv" : "fft_col_fwd", k, dim3(((ngroups + 7) / 8) * 8),
+               dim3(kColThreads), g.lds, data, Ny, W, scale, p, tw, g.LS);
+  });
+}
+
+// The node kernel covers the M = W-1 columns below the (zero) Nyquist column.
+template <class T>
+void node_col(mlc_ctx* ctx, const cpx<T>* fmap, const double* av, const double* ad, int h,
+              cpx<T>* ov, cpx<T>* od, int Nx, int Ny, int W) {
+  Plan p = plan_for(Ny, "column FFT", 8);
+  const cpx<T>* tw = twiddles<T>(ctx, Ny);
+  const double* csy = cos_table(ctx, 2, Ny);
+  const double* phx = cos_table(ctx, 3, Nx);
+  ColGeom g = col_geom<T>(Ny);
+  const size_t coef = sizeof(double) * (size_t)g.CB * h;
+  const int coef_lds = g.lds + coef <= kLdsBytes;
+  const size_t lds = g.lds + (coef_lds ? coef : 0);
+  with_col(g, [&](auto cbc, auto eptc, auto fc) {
+    constexpr int CB = decltype(cbc)::value, EPT = decltype(eptc)::value;
+    constexpr bool FAST = decltype(fc)::value;
+    int ngroups = (W - 1 + CB - 1) / CB;
+    auto k = k_node_col<T, CB, EPT, FAST>;
+    allow_lds(k, lds);
+    MLC_LAUNCH(ctx, "conv_node_col", k, dim3(((ngroups + 7) / 8) * 8), dim3(kColThreads), lds,
+               fmap, av, ad, h, csy, phx, ov, od, Ny, W, p, tw, g.LS, coef_lds);
+  });
+}
+
+}  // namespace
+
+namespace mlc {
+
+template <class T>
+void fft_map_transform(mlc_map* m, const double* d_map) {
+  mlc_ctx* ctx = m->ctx;
+  const int64_t npix = (int64_t)m->Nx * m->Ny;
+  if (m->subtract_mean) {
+    Workspace ws(ctx);
+    int np = 1024;
+    int ip = ws.add(sizeof(double) * np);
+    ws.commit();
+    MLC_LAUNCH(ctx, "map_sum_partial", k_sum_partial, dim3(np), dim3(256), 0, d_map, npix,
+               ws.get<double>(ip));
+    MLC_LAUNCH(ctx, "map_sum_final", k_sum_final, dim3(1), dim3(256), 0, ws.get<double>(ip), np,
+               1.0 / (double)npix, m->mean);
+  } else {
+    hip_check(hipMemsetAsync(m->mean, 0, sizeof(double), ctx->stream), "hipMemsetAsync(mean)");
+  }
+  cpx<T>* F = (cpx<T>*)m->fmap;
+  row_r2c<T, double>(ctx, d_map, F, m->Nx, m->Ny, m->mean);
+  col_fft<T, false>(ctx, F, m->Ny, m->W, (T)(1.0 / (double)npix));
+}
+
+template <class T>
+void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node) {
+  mlc_ctx* ctx = m->ctx;
+  const int n = P.n, W = m->W;
+  const bool want_d = d_node != nullptr;
+  const int h = n / 2;
+  Workspace ws(ctx);
+  int ip = ws.add(sizeof(double) * n * n), idp = ws.add(sizeof(double) * n * n);
+  int ia = ws.add(sizeof(double) * h * W), iad = ws.add(sizeof(double) * h * W);
+  ws.commit();
+  double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
+  double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
+  MLC_LAUNCH(ctx, "profile", k_profile, dim3(1), dim3(1024), 0, P, prof, dprof);
+  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos, dim3(cdiv(W, 256), h), dim3(256), 0, prof, dprof, n,
+             m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr);
+  cpx<T>* G0 = (cpx<T>*)m->g0;
+  cpx<T>* G1 = (cpx<T>*)m->g1;
+  node_col<T>(ctx, (const cpx<T>*)m->fmap, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny, W);
+  if (d_node) row_c2r<T, 2>(ctx, G0, G1, (T*)d_node, m->Nx, m->Ny, m->mean);
+  if (d_value) row_c2r<T, 1>(ctx, G0, nullptr, (T*)d_value, m->Nx, m->Ny, m->mean);
+}
+
+template <class T>
+void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node) {
+  // The reference's structure (gpu_functions.cu:184-208): full-size kernel,
+  // forward FFT, multiply, inverse FFT, each stage followed by a sync.
+  mlc_ctx* ctx = m->ctx;
+  const int n = P.n, W = m->W, Nx = m->Nx, Ny = m->Ny;
+  const int64_t npix = (int64_t)Nx * Ny, nspec = (int64_t)Ny * W;
+  Workspace ws(ctx);
+  int ip = ws.add(sizeof(double) * n * n), idp = ws.add(sizeof(double) * n * n);
+  int ik = ws.add(sizeof(T) * npix), ifk = ws.add(sizeof(cpx<T>) * nspec);
+  int iv = ws.add(sizeof(T) * npix), id = ws.add(sizeof(T) * npix);
+  ws.commit();
+  double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
+  T* ker = ws.get<T>(ik);
+  cpx<T>* fk = ws.get<cpx<T>>(ifk);
+  T* outv = d_value ? (T*)d_value : ws.get<T>(iv);
+  T* outd = ws.get<T>(id);
+  MLC_LAUNCH(ctx, "profile", k_profile, dim3(1), dim3(1024), 0, P, prof, dprof);
+  sync(ctx);
+  const int grid = std::min<int64_t>(cdiv(npix, 256), fill_grid(ctx, 16));
+  for (int pass = 0; pass < (d_node ? 2 : 1); ++pass) {
+    T* dst = pass ? outd : outv;
+    MLC_LAUNCH(ctx, "plain_make_kernel", k_make_kernel<T>, dim3(grid), dim3(256), 0,
+               pass ? dprof : prof, n, Nx, Ny, ker);
+    sync(ctx);
+    row_r2c<T, T>(ctx, ker, fk, Nx, Ny, nullptr);
+    col_fft<T, false>(ctx, fk, Ny, W, (T)1);
+    sync(ctx);
+    MLC_LAUNCH(ctx, "plain_cmul", k_cmul<T>, dim3(grid), dim3(256), 0, (const cpx<T>*)m->fmap, fk,
+               nspec);
+    sync(ctx);
+    col_fft<T, true>(ctx, fk, Ny, W, (T)1);
+    if (pass == 0) row_c2r<T, 1>(ctx, fk, nullptr, dst, Nx, Ny, m->mean);
+    else row_c2r<T, 0>(ctx, fk, nullptr, dst, Nx, Ny, nullptr);
+    sync(ctx);
+  }
+  if (d_node) {
+    MLC_LAUNCH(ctx, "plain_interleave", k_interleave<T>, dim3(grid), dim3(256), 0, outv, outd,
+               (cpx<T>*)d_node, Nx, npix);
+    sync(ctx);
+  }
+}
+
+template <class T>
+void fft_r2c_2d(mlc_ctx* ctx, const T* in, cpx<T>* out, int Nx, int Ny) {
+  row_r2c<T, T>(ctx, in, out, Nx, Ny, nullptr);
+  col_fft<T, false>(ctx, out, Ny, Nx / 2 + 1, (T)1);
+}
+
+// inverse of a half spectrum held in `work` (overwritten) into `out`
+template <class T>
+void fft_c2r_2d(mlc_ctx* ctx, cpx<T>* work, T* out, int Nx, int Ny) {
+  col_fft<T, true>(ctx, work, Ny, Nx / 2 + 1, (T)1);
+  row_c2r<T, 0>(ctx, work, nullptr, out, Nx, Ny, nullptr);
+}
+
+#define MLC_FFT_INSTANTIATE(T)                                                               \
+  template void fft_map_transform<T>(mlc_map*, const double*);                               \
+  template void fft_convolve_fused<T>(mlc_map*, const ProfileParams&, void*, void*);         \
+  template void fft_convolve_plain<T>(mlc_map*, const ProfileParams&, void*, void*);         \
+  template void fft_r2c_2d<T>(mlc_ctx*, const T*, cpx<T>*, int, int);                        \
+  template void fft_c2r_2d<T>(mlc_ctx*, cpx<T>*, T*, int, int);
+
+}  // namespace mlc

@@ -15,15 +15,23 @@
 // and the gradient of chi2 with respect to the seven parameters follows by
 // the chain rule through the same bilinear weights.
 //
-// Layout: one wavefront per curve (lanes stride over its epochs, all bands
-// concatenated), four curves per workgroup, wavefront reductions at the end.
-// A curve that leaves the pixel-centre hull or meets mu <= 0 gets chi2 = +inf
-// and a zero gradient.
+// Layout.  The node maps are in 4 x 4 pixel tiles (mlc_tiles.hpp).  The
+// fused path first partitions the curves by profile into 64-byte records
+// (origin, velocity components, radius law, offset), then runs one launch
+// per (profile p, band b): a group of lanes per curve strides over the
+// band-b epochs, and the group's FP64 sums go to the curve's accumulator
+// (the first band writes it, later bands add in band order).  Only the node
+// maps of (p, b) are read during a launch, so they stay in the last-level
+// cache; every accumulator has one writer per launch, so the result does not
+// depend on scheduling.  A curve that leaves the pixel-centre hull or meets
+// mu <= 0 gets chi2 = +inf and a zero gradient.
 #include <cmath>
 #include <vector>
 
 #include "mlc_fft_core.hpp"
 #include "mlc_internal.hpp"
+#include "mlc_math.hpp"
+#include "mlc_tiles.hpp"
 
 using namespace mlc;
 
@@ -43,10 +51,15 @@ struct Tap {
   T v, dx, dy;  // bilinear value and its x / y derivative
 };
 
+// Bilinear taps at pixel (ix, iy)..(ix+1, iy+1) of a tiled node map.
 template <class T>
-__device__ __forceinline__ void taps(const cpx<T>* __restrict__ m, int Nx, T fx, T fy, Tap<T>* M,
-                                     Tap<T>* D) {
-  const cpx<T> a = m[0], b = m[1], c = m[Nx], d = m[Nx + 1];
+__device__ __forceinline__ void taps(const cpx<T>* __restrict__ m, int Nx, int ix, int iy, T fx,
+                                     T fy, Tap<T>* M, Tap<T>* D) {
+  const long long o00 = tile_offset(ix, iy, Nx);
+  const long long o01 = (ix & 3) != 3 ? o00 + 1 : tile_offset(ix + 1, iy, Nx);
+  const long long o10 = (iy & 3) != 3 ? o00 + 4 : tile_offset(ix, iy + 1, Nx);
+  const long long o11 = (ix & 3) != 3 ? o10 + 1 : tile_offset(ix + 1, iy + 1, Nx);
+  const cpx<T> a = m[o00], b = m[o01], c = m[o10], d = m[o11];
   const T gx = (T)1 - fx, gy = (T)1 - fy;
   M->v = gy * (gx * a.x + fx * b.x) + fy * (gx * c.x + fx * d.x);
   M->dx = gy * (b.x - a.x) + fy * (d.x - c.x);
@@ -74,7 +87,6 @@ __device__ __forceinline__ Sample sample_at(const cpx<T>* __restrict__ nodes, si
   }
   int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
   const T fx = (T)(x - ix), fy = (T)(y - iy);
-  const size_t pix = (size_t)iy * Nx + ix;
   const int noff = tab.off[pb], ncnt = tab.cnt[pb];
   int k = 0;
   bool taylor = true;
@@ -91,7 +103,7 @@ __device__ __forceinline__ Sample sample_at(const cpx<T>* __restrict__ nodes, si
     }
   }
   Tap<T> M0, D0;
-  taps<T>(nodes + mapsz * (noff + k) + pix, Nx, fx, fy, &M0, &D0);
+  taps<T>(nodes + mapsz * (noff + k), Nx, ix, iy, fx, fy, &M0, &D0);
   if (taylor) {
     const T dr = (T)(r - tab.rad[noff + k]);
     s.mu = (double)(M0.v + D0.v * dr);
@@ -100,7 +112,7 @@ __device__ __forceinline__ Sample sample_at(const cpx<T>* __restrict__ nodes, si
     s.mr = (double)D0.v;
   } else {
     Tap<T> M1, D1;
-    taps<T>(nodes + mapsz * (noff + k + 1) + pix, Nx, fx, fy, &M1, &D1);
+    taps<T>(nodes + mapsz * (noff + k + 1), Nx, ix, iy, fx, fy, &M1, &D1);
     const T S = (T)sr, S2 = S * S, S3 = S2 * S, H = (T)h;
     const T h00 = 2 * S3 - 3 * S2 + 1, h10 = (S3 - 2 * S2 + S) * H;
     const T h01 = -2 * S3 + 3 * S2, h11 = (S3 - S2) * H;
@@ -115,70 +127,324 @@ __device__ __forceinline__ Sample sample_at(const cpx<T>* __restrict__ nodes, si
   return s;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// ---- fused path
+// Curves are grouped by profile (counting partition, one bucket per profile
+// plus bucket P for invalid profile indices) into 64-byte records in bucket
+// order; the band launches and the finish kernel address the per-curve
+// accumulators by record position.
+struct alignas(64) CurveRec {
+  double x0, y0, vct, vst, r0, w, dm;  // vct, vst = v cos(theta), v sin(theta)
+  int c, o;                            // curve index, observation set
+};
+static_assert(sizeof(CurveRec) == 64, "one record per 64 bytes");
+
+constexpr int kPartChunk = 2048;    // curves per partition block (256 threads x 8)
+constexpr int kMaxProfiles = 4096;  // bucket counters live in LDS
+constexpr int kMaxBandNodes = 16;   // radius nodes of one (profile, band), kernel argument
+#ifndef MLC_LC_LANES
+#define MLC_LC_LANES 16
+#endif
+constexpr int kLanes = MLC_LC_LANES;  // lanes per curve in a band launch
+static_assert(kLanes >= 8 && kLanes <= 64 && (kLanes & (kLanes - 1)) == 0, "lanes per curve");
+
+// Per-block bucket counts, stored bucket-major: bh[q * nblk + block].
+__global__ __launch_bounds__(256) void k_part_count(const int* __restrict__ prof, int n, int P,
+                                                    int* __restrict__ bh) {
+  extern __shared__ int h[];
+  for (int q = threadIdx.x; q <= P; q += 256) h[q] = 0;
+  __syncthreads();
+  const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
+  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
+    const int p = prof[c];
+    atomicAdd(&h[(p >= 0 && p < P) ? p : P], 1);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q <= P; q += 256) bh[(size_t)q * gridDim.x + blockIdx.x] = h[q];
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_lc_chi2_grad(
-    const cpx<T>* __restrict__ nodes, int Nx, int Ny, int P, int B, NodeTab tab, int ncurves,
-    const double* __restrict__ params, const int* __restrict__ prof, const int* __restrict__ obs,
+// Exclusive scan of the m = (P+1)*nblk block counts in bucket-major order (one
+// workgroup), then start[q] / cnt[q] of every bucket.
+__global__ __launch_bounds__(1024) void k_part_scan(int* __restrict__ bh, int m, int nblk, int P, int n,
+                                                    int* __restrict__ start, int* __restrict__ cnt) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  const int64_t seg = ((int64_t)m + 1023) / 1024;
+  const int64_t a = min((int64_t)m, t * seg), e = min((int64_t)m, a + seg);
+  int sum = 0;
+  for (int64_t i = a; i < e; ++i) sum += bh[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - sum;
+  for (int64_t i = a; i < e; ++i) {
+    const int v = bh[i];
+    bh[i] = run;
+    run += v;
+  }
+  __syncthreads();
+  for (int q = t; q <= P; q += 1024) {
+    const int st = bh[(size_t)q * nblk];
+    start[q] = st;
+    cnt[q] = (q < P ? bh[(size_t)(q + 1) * nblk] : n) - st;
+  }
+}
+
+// Records of the block's curves at the block's bucket offsets (the order
+// inside one block's share of a bucket is whatever the LDS counter gives; no
+// result depends on it).
+__global__ __launch_bounds__(256) void k_part_fill(const int* __restrict__ prof, int n, int P,
+                                                   const int* __restrict__ bh,
+                                                   const double* __restrict__ params,
+                                                   const int* __restrict__ obs,
+                                                   CurveRec* __restrict__ rec) {
+  extern __shared__ int h[];
+  for (int q = threadIdx.x; q <= P; q += 256) h[q] = bh[(size_t)q * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
+  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
+    const int p = prof[c];
+    const int pos = atomicAdd(&h[(p >= 0 && p < P) ? p : P], 1);
+    const double* Pc = params + (size_t)c * 7;
+    double sn, cs;
+    sincos(Pc[2], &sn, &cs);
+    CurveRec r;
+    r.x0 = Pc[0];
+    r.y0 = Pc[1];
+    r.vct = Pc[3] * cs;
+    r.vst = Pc[3] * sn;
+    r.r0 = Pc[4];
+    r.w = Pc[5];
+    r.dm = Pc[6];
+    r.c = c;
+    r.o = obs[c];
+    rec[pos] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_inv_sig(const double* __restrict__ sig, int nepoch,
+                                                 double* __restrict__ isg) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < nepoch) isg[j] = 1.0 / sig[j];
+}
+
+// The radius nodes of one (profile, band), passed by value.
+struct BandNodes {
+  int cnt;                     // >= 1
+  double bsc;                  // band_scale[b]
+  double rad[kMaxBandNodes];   // ascending
+};
+
+// 32-bit pair offset of pixel (x, y) in a tiled map (Nx * Ny < 2^31).
+__device__ __forceinline__ int tile32(int x, int y, int Nx) {
+  return (((y >> 2) * (Nx >> 2) + (x >> 2)) << 4) + ((y & 3) << 2) + (x & 3);
+}
+
+// Sum of v over the LPC lanes of a group, 8 values at once: three halving
+// exchanges leave lane li with value index (li / (LPC/8)) & 7, the remaining
+// exchanges finish that value.  Returns the lane's value.
+template <int LPC>
+__device__ __forceinline__ double group_sum8(double (&v)[8], int li) {
+  constexpr int o1 = LPC / 2, o2 = LPC / 4, o3 = LPC / 8;
+  double a[4], c[2];
+  {
+    const bool up = li & o1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double send = up ? v[q] : v[q + 4], keep = up ? v[q + 4] : v[q];
+      a[q] = keep + __shfl_xor(send, o1, 64);
+    }
+  }
+  {
+    const bool up = li & o2;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const double send = up ? a[q] : a[q + 2], keep = up ? a[q + 2] : a[q];
+      c[q] = keep + __shfl_xor(send, o2, 64);
+    }
+  }
+  const bool up = li & o3;
+  double r = (up ? c[1] : c[0]) + __shfl_xor(up ? c[0] : c[1], o3, 64);
+#pragma unroll
+  for (int o = o3 / 2; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+  return r;
+}
+
+// One (profile p, band b) launch.  A group of kLanes lanes per curve record
+// (64/kLanes curves per wavefront, grid-stride over the records of bucket
+// p), the band-b epochs strided across the group.  Per epoch:
+//   mu, dmu/dx, dmu/dy, dmu/dr  from the bilinear taps of the nodes (T)
+//   rho = (mobs - dm + 2.5 log10 mu) / sig,  q = rho / sig        (FP64)
+//   dc = d chi2 / d mu = 5 q / (ln 10 mu)
+// and the group sums, in FP64,
+//   [0] rho^2  [1] dc mx  [2] dc my  [3] dc t mx  [4] dc t my
+//   [5] bsc dc mr  [6] bsc dc t mr  [7] -2 q
+// go to the record's accumulator (band 0 writes, later bands add).
+template <class T, bool FIRST>
+__global__ __launch_bounds__(256) void k_lc_band(
+    const cpx<T>* __restrict__ base, int Nx, int Ny, BandNodes bn, int p, int b, int B,
+    const CurveRec* __restrict__ rec, const int* __restrict__ start, const int* __restrict__ cnt,
     int nobs, const int* __restrict__ obs_off, int nepoch, const double* __restrict__ tt,
-    const double* __restrict__ mobs, const double* __restrict__ sig, double* __restrict__ chi2,
-    double* __restrict__ grad) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= ncurves) return;  // whole wavefront
+    const double* __restrict__ mobs, const double* __restrict__ isg, double* __restrict__ acc,
+    int* __restrict__ bad) {
+  constexpr int GPW = 64 / kLanes;
+  constexpr double kMag = -1.08573620475812959810;  // -2.5 / ln 10
+  constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
+  __shared__ double srad[kMaxBandNodes];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < kMaxBandNodes; ++q) srad[q] = bn.rad[q];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, gi = lane / kLanes, li = lane % kLanes;
+  const int pos0 = start[p], ncur = cnt[p];
   const size_t mapsz = (size_t)Nx * Ny;
-  const double* Pc = params + (size_t)c * 7;
-  const double x0 = Pc[0], y0 = Pc[1], th = Pc[2], v = Pc[3], r0 = Pc[4], w = Pc[5], dm = Pc[6];
-  const double ct = cos(th), st = sin(th);
-  const int p = prof[c], o = obs[c];
-  double acc = 0.0, g[7] = {0, 0, 0, 0, 0, 0, 0};
-  bool bad = !(p >= 0 && p < P && o >= 0 && o < nobs);
-  if (!bad) {
-    const int* off = obs_off + (size_t)o * B;
-    const int e0 = off[0], e1 = off[B];
-    bad = !(e0 >= 0 && e1 <= nepoch && e0 <= e1);
-    if (!bad) {
-      for (int j = e0 + lane; j < e1; j += 64) {
-        int b = 0;
-        while (b < B - 1 && j >= off[b + 1]) ++b;
+  const double bsc = bn.bsc, xmax = (double)(Nx - 1), ymax = (double)(Ny - 1);
+  const int nn = bn.cnt;
+  const double rlo = srad[0], rhi = srad[nn - 1];
+  const int nwaves = gridDim.x * 4;
+  for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w * GPW < ncur; w += nwaves) {  // uniform per wave
+    const int i = w * GPW + gi;
+    const bool act = i < ncur;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool flag = false;
+    if (act) {
+      const CurveRec R = rec[pos0 + i];
+      const int o = R.o;
+      const bool okobs = o >= 0 && o < nobs;
+      const int e0 = okobs ? obs_off[(size_t)o * B + b] : 0;
+      const int e1 = okobs ? obs_off[(size_t)o * B + b + 1] : 0;
+      flag = !(okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1);
+      const int jend = flag ? e0 : e1;
+      for (int j = e0 + li; j < jend; j += kLanes) {
         const double te = tt[j];
-        const double x = x0 + v * ct * te, y = y0 + v * st * te;
-        const double bsc = tab.bs[b];
-        const double r = bsc * (r0 + w * te);
-        Sample s = sample_at<T>(nodes, mapsz, Nx, Ny, tab, p * B + b, x, y, r);
-        if (!s.ok) {
-          bad = true;
-          break;
+        const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
+        const double r = bsc * fma(R.w, te, R.r0);
+        const bool in = x >= 0.0 && x <= xmax && y >= 0.0 && y <= ymax;
+        const double xs = in ? x : 0.0, ys = in ? y : 0.0;
+        const int ix = min((int)xs, Nx - 2), iy = min((int)ys, Ny - 2);
+        const T fx = (T)(xs - ix), fy = (T)(ys - iy), gx = (T)1 - fx, gy = (T)1 - fy;
+        const int o00 = tile32(ix, iy, Nx);
+        const int o01 = (ix & 3) != 3 ? o00 + 1 : tile32(ix + 1, iy, Nx);
+        const int o10 = (iy & 3) != 3 ? o00 + 4 : tile32(ix, iy + 1, Nx);
+        const int o11 = (ix & 3) != 3 ? o10 + 1 : tile32(ix + 1, iy + 1, Nx);
+        // corner values u (mu) and v (dmu/dr) from one or two radius nodes
+        T u[4], v[4];
+        if (nn == 1) {  // uniform: first-order Taylor step about the only node
+          const T dr = (T)(r - rlo);
+          const cpx<T> c0 = base[o00], c1 = base[o01], c2 = base[o10], c3 = base[o11];
+          u[0] = c0.x + dr * c0.y; v[0] = c0.y;
+          u[1] = c1.x + dr * c1.y; v[1] = c1.y;
+          u[2] = c2.x + dr * c2.y; v[2] = c2.y;
+          u[3] = c3.x + dr * c3.y; v[3] = c3.y;
+        } else {
+          // node interval k, k+1; Taylor about the end node outside [rlo, rhi]
+          int k = 0;
+          while (k < nn - 2 && srad[k + 1] <= r) ++k;
+          const bool lo = r <= rlo, hi = r >= rhi;
+          const int ka = hi ? nn - 1 : k, kb = hi ? nn - 1 : k + 1;
+          const double ra = srad[ka], h = srad[kb] - ra;
+          T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
+          if (lo || hi) {
+            const T dr = (T)(r - ra);
+            wa = 1; wda = dr; wb = 0; wdb = 0;
+            za = 0; zda = 1; zb = 0; zdb = 0;
+          } else {
+            const T S = (T)((r - ra) / h), S2 = S * S, S3 = S2 * S, H = (T)h, iH = (T)(1.0 / h);
+            wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
+            wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
+            za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
+            zb = (-6 * S2 + 6 * S) * iH; zdb = 3 * S2 - 2 * S;
+          }
+          const cpx<T>* ma = base + mapsz * ka;
+          const cpx<T>* mb = base + mapsz * kb;
+          const int off[4] = {o00, o01, o10, o11};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const cpx<T> A = ma[off[q]], Bq = mb[off[q]];
+            u[q] = wa * A.x + wda * A.y + wb * Bq.x + wdb * Bq.y;
+            v[q] = za * A.x + zda * A.y + zb * Bq.x + zdb * Bq.y;
+          }
         }
-        const double sg = sig[j];
-        const double m = -2.5 * log10(s.mu) + dm;
-        const double rho = (mobs[j] - m) / sg;
-        acc += rho * rho;
-        const double dc = 2.0 * rho * 2.5 / (kLn10 * s.mu * sg);
-        g[0] += dc * s.mx;
-        g[1] += dc * s.my;
-        g[2] += dc * (s.mx * (-v * st * te) + s.my * (v * ct * te));
-        g[3] += dc * (s.mx * ct * te + s.my * st * te);
-        g[4] += dc * s.mr * bsc;
-        g[5] += dc * s.mr * bsc * te;
-        g[6] += -2.0 * rho / sg;
+        const T mu_t = gy * (gx * u[0] + fx * u[1]) + fy * (gx * u[2] + fx * u[3]);
+        const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
+        const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
+        const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
+        const double mu = (double)mu_t;
+        flag |= !(in && mu > 0.0 && mu < INFINITY);
+        const double is = isg[j];
+        const double rho = (mobs[j] - fma(kMag, ln_pos(mu), R.dm)) * is;
+        const double q = rho * is;
+        double rmu;
+        if constexpr (sizeof(T) == 4) rmu = (double)__builtin_amdgcn_rcpf(mu_t);
+        else rmu = 1.0 / mu;
+        const double dc = kDc * q * rmu, tdc = dc * te;
+        const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
+        s[0] = fma(rho, rho, s[0]);
+        s[1] = fma(dc, mx, s[1]);
+        s[2] = fma(dc, my, s[2]);
+        s[3] = fma(tdc, mx, s[3]);
+        s[4] = fma(tdc, my, s[4]);
+        s[5] = fma(dc, mr, s[5]);
+        s[6] = fma(tdc, mr, s[6]);
+        s[7] += q;
+      }
+    }
+    const double val = group_sum8<kLanes>(s, li);
+    const unsigned long long bm = __ballot(flag);
+    constexpr unsigned long long gmask = kLanes == 64 ? ~0ull : ((1ull << (kLanes & 63)) - 1);
+    const bool gbad = ((bm >> (gi * kLanes)) & gmask) != 0;
+    if (act && li % (kLanes / 8) == 0) {
+      const int vi = li / (kLanes / 8);
+      const double sc = vi == 7 ? -2.0 : (vi == 5 || vi == 6) ? bsc : 1.0;
+      double* a = acc + (size_t)(pos0 + i) * 8 + vi;
+      if (FIRST) *a = sc * val;
+      else *a += sc * val;
+      if (vi == 0) {
+        if (FIRST) bad[pos0 + i] = gbad;
+        else bad[pos0 + i] |= gbad;
       }
     }
   }
-  const bool any_bad = __ballot(bad) != 0;
-  acc = wave_sum(acc);
+}
+
+// chi2 and the 7-parameter gradient of every curve from its accumulator:
+// d/dtheta = v (cos(theta) [4] - sin(theta) [3]), d/dv = cos(theta) [3] +
+// sin(theta) [4]; records of bucket P (invalid profile) and flagged ones
+// give (+inf, 0).
+__global__ __launch_bounds__(256) void k_lc_finish(int n, int P, const int* __restrict__ start,
+                                                   const CurveRec* __restrict__ rec,
+                                                   const double* __restrict__ params,
+                                                   const double* __restrict__ acc,
+                                                   const int* __restrict__ bad,
+                                                   double* __restrict__ chi2, double* __restrict__ grad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = rec[i].c;
+  const bool ok = i < start[P] && !bad[i];
+  double* g = grad + (size_t)c * 7;
+  if (!ok) {
+    chi2[c] = INFINITY;
 #pragma unroll
-  for (int q = 0; q < 7; ++q) g[q] = wave_sum(g[q]);
-  if (lane == 0) {
-    chi2[c] = any_bad ? INFINITY : acc;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) grad[(size_t)c * 7 + q] = any_bad ? 0.0 : g[q];
+    for (int q = 0; q < 7; ++q) g[q] = 0.0;
+    return;
   }
+  const double* a = acc + (size_t)i * 8;
+  double sn, cs;
+  sincos(params[(size_t)c * 7 + 2], &sn, &cs);
+  const double v = params[(size_t)c * 7 + 3];
+  chi2[c] = a[0];
+  g[0] = a[1];
+  g[1] = a[2];
+  g[2] = v * (cs * a[4] - sn * a[3]);
+  g[3] = cs * a[3] + sn * a[4];
+  g[4] = a[5];
+  g[5] = a[6];
+  g[6] = a[7];
 }
 
 // ---- plain path: stage 1 writes (mu, dmu/dx, dmu/dy, dmu/dr) per sample,
@@ -289,9 +555,10 @@ struct Args {
 
 void validate(const Args& a) {
   MLC_REQUIRE(a.precision == MLC_F32 || a.precision == MLC_F64, "lc_chi2_grad: bad precision");
-  MLC_REQUIRE(a.Nx >= 2 && a.Ny >= 2 && a.P >= 1 && a.B >= 1 && a.nnodes >= 1 && a.ncurves >= 0 &&
+  MLC_REQUIRE(a.Nx >= 4 && a.Ny >= 4 && a.P >= 1 && a.B >= 1 && a.nnodes >= 1 && a.ncurves >= 0 &&
                   a.nobs >= 1 && a.nepoch >= 0,
               "lc_chi2_grad: bad sizes");
+  MLC_REQUIRE(a.Nx % 4 == 0 && a.Ny % 4 == 0, "lc_chi2_grad: node maps need Nx, Ny divisible by 4");
   MLC_REQUIRE(a.nodes && a.node_off && a.node_cnt && a.node_rad && a.band_scale,
               "lc_chi2_grad: null node table");
   MLC_REQUIRE(a.ncurves == 0 || (a.params && a.prof && a.obs && a.obs_off && a.chi2 && a.grad),
@@ -316,22 +583,61 @@ NodeTab upload_tab(mlc_ctx* ctx, const Args& a, Workspace& ws, int io, int ic, i
 
 void run_fused(mlc_ctx* ctx, const Args& a) {
   validate(a);
+  MLC_REQUIRE(a.P <= kMaxProfiles, "lc_chi2_grad: at most 4096 profiles");
+  MLC_REQUIRE((int64_t)a.Nx * a.Ny <= INT32_MAX, "lc_chi2_grad: map too large for 32-bit offsets");
+  for (int i = 0; i < a.P * a.B; ++i)
+    MLC_REQUIRE(a.node_cnt[i] <= kMaxBandNodes, "lc_chi2_grad: at most 16 radius nodes per (profile, band)");
   if (a.ncurves == 0) return;
+  const int P = a.P, B = a.B, n = a.ncurves;
+  const int nblk = (int)cdiv(n, kPartChunk);
+  const int64_t m = (int64_t)(P + 1) * nblk;
+  MLC_REQUIRE(m <= INT32_MAX, "lc_chi2_grad: too many curves x profiles");
   Workspace ws(ctx);
-  int io = ws.add(sizeof(int) * a.P * a.B), ic = ws.add(sizeof(int) * a.P * a.B),
-      ir = ws.add(sizeof(double) * a.nnodes), ib = ws.add(sizeof(double) * a.B);
+  const int ibh = ws.add(sizeof(int) * m), ist = ws.add(sizeof(int) * 2 * (P + 1)),
+            irec = ws.add(sizeof(CurveRec) * (size_t)n), iacc = ws.add(sizeof(double) * 8 * (size_t)n),
+            ibad = ws.add(sizeof(int) * (size_t)n), iisg = ws.add(sizeof(double) * std::max(a.nepoch, 1));
   ws.commit();
-  NodeTab tab = upload_tab(ctx, a, ws, io, ic, ir, ib);
-  dim3 grid(cdiv(a.ncurves, 4));
-  if (a.precision == MLC_F64)
-    MLC_LAUNCH(ctx, "lc_chi2_grad", k_lc_chi2_grad<double>, grid, dim3(256), 0,
-               (const cpx<double>*)a.nodes, a.Nx, a.Ny, a.P, a.B, tab, a.ncurves, a.params, a.prof,
-               a.obs, a.nobs, a.obs_off, a.nepoch, a.t, a.mobs, a.sig, a.chi2, a.grad);
-  else
-    MLC_LAUNCH(ctx, "lc_chi2_grad", k_lc_chi2_grad<float>, grid, dim3(256), 0,
-               (const cpx<float>*)a.nodes, a.Nx, a.Ny, a.P, a.B, tab, a.ncurves, a.params, a.prof,
-               a.obs, a.nobs, a.obs_off, a.nepoch, a.t, a.mobs, a.sig, a.chi2, a.grad);
-  sync(ctx);  // the node tables were staged from host memory of this call
+  int* bh = ws.get<int>(ibh);
+  int *start = ws.get<int>(ist), *cnt = start + (P + 1);
+  CurveRec* rec = ws.get<CurveRec>(irec);
+  double* acc = ws.get<double>(iacc);
+  int* bad = ws.get<int>(ibad);
+  double* isg = ws.get<double>(iisg);
+  const size_t hl = sizeof(int) * (P + 1);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(256), hl, a.prof, n, P, bh);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_scan, dim3(1), dim3(1024), 0, bh, (int)m, nblk, P, n, start, cnt);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, a.prof, n, P, bh, a.params,
+             a.obs, rec);
+  if (a.nepoch > 0)
+    MLC_LAUNCH(ctx, "lc_partition", k_inv_sig, dim3(cdiv(a.nepoch, 256)), dim3(256), 0, a.sig, a.nepoch,
+               isg);
+  constexpr int kCurvesPerBlock = 4 * (64 / kLanes);
+  const int gb = std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, 8));
+  const size_t mapsz = (size_t)a.Nx * a.Ny;
+  const size_t esz = a.precision == MLC_F64 ? 2 * sizeof(double) : 2 * sizeof(float);
+  for (int p = 0; p < P; ++p)
+    for (int b = 0; b < B; ++b) {
+      const int pb = p * B + b;
+      BandNodes bn{};
+      bn.cnt = a.node_cnt[pb];
+      bn.bsc = a.band_scale[b];
+      for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
+      const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
+#define MLC_BAND(TT, F)                                                                              \
+  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F>), dim3(gb), dim3(256), 0, (const cpx<TT>*)base,  \
+             a.Nx, a.Ny, bn, p, b, B, rec, start, cnt, a.nobs, a.obs_off, a.nepoch, a.t, a.mobs, isg, \
+             acc, bad)
+      if (a.precision == MLC_F64) {
+        if (b == 0) MLC_BAND(double, true);
+        else MLC_BAND(double, false);
+      } else {
+        if (b == 0) MLC_BAND(float, true);
+        else MLC_BAND(float, false);
+      }
+#undef MLC_BAND
+    }
+  MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, P, start, rec, a.params,
+             acc, bad, a.chi2, a.grad);
 }
 
 void run_plain(mlc_ctx* ctx, const Args& a) {

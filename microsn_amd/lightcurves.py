@@ -25,36 +25,47 @@ NPARAM = 7
 class NodeSet:
     """Convolved (mu, dmu/drhalf) maps at a few radii per (profile, band).
 
-    nodes[k] is the node map k, shape (Ny, Nx, 2); table[p][b] lists the node
-    indices of profile p in band b with ascending radii."""
+    nodes[k] is node map k, shape (Ny, Nx, 2), stored in the library's 4 x 4
+    pixel tiles (rowmajor() gives the plain layout); node_off/node_cnt give
+    the nodes of profile p in band b with ascending radii node_rad."""
 
-    def __init__(self, ctx, spectrum: MapSpectrum, shapes, radii, band_scale, pix=1.0, plain=False):
+    def __init__(self, ctx, spectrum: MapSpectrum, shapes, radii, band_scale, pix=1.0, plain=False,
+                 compute=True):
         """shapes: P profile shapes; radii[p][b]: ascending list of node radii
-        (half-light radius in pixels) for profile p, band b."""
-        self.ctx = ctx
+        (half-light radius in pixels) for profile p, band b.  With
+        compute=False the maps are only allocated (see compute())."""
+        self.ctx, self.spectrum, self.pix = ctx, spectrum, float(pix)
+        self.shapes = [_shape_id(s) for s in shapes]
         self.P, self.B = len(shapes), len(band_scale)
         self.Ny, self.Nx = spectrum.Ny, spectrum.Nx
         self.prec = spectrum.prec
-        rad, off, cnt = [], [], []
+        rad, off, cnt, shp = [], [], [], []
         for p in range(self.P):
             for b in range(self.B):
                 rs = list(radii[p][b])
                 off.append(len(rad))
                 cnt.append(len(rs))
                 rad.extend(float(r) for r in rs)
+                shp.extend([self.shapes[p]] * len(rs))
         self.node_off = np.array(off, np.int32)
         self.node_cnt = np.array(cnt, np.int32)
         self.node_rad = np.array(rad, np.float64)
+        self.node_shape = shp
         self.band_scale = np.ascontiguousarray(band_scale, np.float64)
         self.nodes = torch.empty((len(rad), self.Ny, self.Nx, 2), dtype=spectrum.dtype, device=ctx.device)
-        k = 0
-        for p in range(self.P):
-            for b in range(self.B):
-                for r in radii[p][b]:
-                    f = lib().mlc_map_convolve_plain if plain else lib().mlc_map_convolve
-                    ctx.check(f(spectrum.h, _shape_id(shapes[p]), float(r), float(pix), 0.0, 1.0, None,
-                                ptr(self.nodes[k])))
-                    k += 1
+        if compute:
+            self.compute(plain)
+
+    def rowmajor(self):
+        """All node maps as a row-major (nnodes, Ny, Nx, 2) tensor."""
+        from .api import untile
+        return untile(self.nodes)
+
+    def compute(self, plain=False):
+        """(Re)convolve every node map from the spectrum's current map."""
+        f = lib().mlc_map_convolve_plain if plain else lib().mlc_map_convolve
+        for k, (s, r) in enumerate(zip(self.node_shape, self.node_rad)):
+            self.ctx.check(f(self.spectrum.h, s, float(r), self.pix, 0.0, 1.0, None, ptr(self.nodes[k])))
 
 
 def _shape_id(s):

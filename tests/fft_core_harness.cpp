@@ -9,44 +9,53 @@
 
 using namespace mlc;
 
+// Runs all stages ping-ponging between two buffers.  With `quarter` the
+// twiddles come from a quarter-wave table, as in the gfx950 kernels.
 template <class T, bool INV>
-static void run(std::vector<cpx<T>>& a, int L) {
+static void run(std::vector<cpx<T>>& a, int L, bool quarter = false) {
   Plan p;
   if (!make_plan(L, &p)) return;
-  std::vector<cpx<T>> tw(L), b(L);
+  std::vector<cpx<T>> tw(L);
   for (int m = 0; m < L; ++m) {
     long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / L;
     tw[m] = mk<T>((T)std::cos(ang), (T)std::sin(ang));
   }
+  std::vector<cpx<T>> x(a), y(L);
+  TableTw<T> full{tw.data()};
+  QuarterTw<T> qtw{tw.data(), L / 4};
   int Ns = 1;
   for (int s = 0; s < p.n; ++s) {
-    int R = p.r[s];
+    const int R = p.r[s];
     for (int j = 0; j < L / R; ++j) {
-      cpx<T> v[8];
+      cpx<T> v[16];
+#define H_CASE(RR)                                                                      \
+  case RR:                                                                              \
+    if (quarter && L % 4 == 0) stage_load<T, RR, INV>(x.data(), L, Ns, j, qtw, v);     \
+    else stage_load<T, RR, INV>(x.data(), L, Ns, j, full, v);                          \
+    stage_store<T, RR>(y.data(), Ns, j, v);                                            \
+    break;
       switch (R) {
-        case 2: stage_load<T, 2, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 2>(b.data(), Ns, j, v); break;
-        case 3: stage_load<T, 3, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 3>(b.data(), Ns, j, v); break;
-        case 4: stage_load<T, 4, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 4>(b.data(), Ns, j, v); break;
-        case 5: stage_load<T, 5, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 5>(b.data(), Ns, j, v); break;
-        case 8: stage_load<T, 8, INV>(a.data(), L, Ns, j, tw.data(), v); stage_store<T, 8>(b.data(), Ns, j, v); break;
+        H_CASE(2) H_CASE(3) H_CASE(4) H_CASE(5) H_CASE(8) H_CASE(16)
       }
+#undef H_CASE
     }
-    a.swap(b);
+    x.swap(y);
     Ns *= R;
   }
+  a = x;
 }
 
 extern "C" {
 
 // In-place complex FFT of n interleaved doubles; returns -1 for an
 // unsupported length.
-int h_fft(double* data, int L, int inverse) {
+int h_fft(double* data, int L, int inverse, int quarter) {
   Plan p;
   if (!make_plan(L, &p)) return -1;
   std::vector<cpx<double>> a(L);
   for (int i = 0; i < L; ++i) a[i] = mk(data[2 * i], data[2 * i + 1]);
-  if (inverse) run<double, true>(a, L);
-  else run<double, false>(a, L);
+  if (inverse) run<double, true>(a, L, quarter != 0);
+  else run<double, false>(a, L, quarter != 0);
   for (int i = 0; i < L; ++i) {
     data[2 * i] = a[i].x;
     data[2 * i + 1] = a[i].y;

@@ -52,7 +52,7 @@ def _params(rng, n, Ny, Nx, P, nobs, tmax=30.0):
 
 
 def _oracle(orc, nodes, sets, prm, prof, obsi):
-    maps = nodes.nodes.double().cpu().numpy()
+    maps = nodes.rowmajor().double().cpu().numpy()
     off = [0]
     t, mo, sg = [], [], []
     for bands in sets:
