@@ -14,7 +14,7 @@ extern "C" {
 // locations_on_grid (locations.cpp:6-14).  The spacing is an integer
 // division, as in the reference, then used as a double.
 int mlc_locations_on_grid(int Nside, int Nmap, int Noff, int* x, int* y) {
-  if (Nside <= 0 || !x || !y) return 1;
+  if (Nside <= 0 || Noff < 0 || Nmap - 2 * Noff <= 0 || !x || !y) return 1;
   const double step = (double)((Nmap - 2 * Noff) / Nside);
   for (int row = 0; row < Nside; ++row) {
     const int iy = Noff + (int)std::floor(step / 2.0 + row * step);
@@ -30,7 +30,7 @@ int mlc_locations_on_grid(int Nside, int Nmap, int Noff, int* x, int* y) {
 // uniform integer in [0, Nmap - 2 Noff], x drawn before y for each location.
 // The integer distribution is libstdc++'s (the reference's toolchain).
 int mlc_locations_random(int Nloc, int Nmap, int Noff, int seed, int* x, int* y) {
-  if (Nloc < 0 || (Nloc > 0 && (!x || !y)) || Nmap - 2 * Noff < 0) return 1;
+  if (Nloc < 0 || (Nloc > 0 && (!x || !y)) || Noff < 0 || Nmap - 2 * Noff < 0) return 1;
   std::mt19937 gen(seed);
   std::uniform_int_distribution<unsigned> pick(0, (unsigned)(Nmap - 2 * Noff));
   for (int i = 0; i < Nloc; ++i) {
@@ -104,7 +104,7 @@ int mlc_setup_chi2(double MA, double MB, double V, double Dfac, int Nd, const do
 
 // Mpd::Mpd(Nbins, min, max, type) bin right edges (mpd.cpp:4-21).
 int mlc_mpd_bins(int Nbins, double min, double max, int log_scale, double* bins) {
-  if (Nbins <= 0 || !bins) return 1;
+  if (Nbins <= 0 || !bins || !(max > min)) return 1;
   if (log_scale) {
     if (!(min > 0) || !(max > 0)) return 1;
     const double lo = std::log10(min), step = (std::log10(max) - lo) / Nbins;
