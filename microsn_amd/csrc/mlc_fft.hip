@@ -88,6 +88,7 @@ int mlc_map_create(mlc_ctx* ctx, int precision, const double* d_map, int Nx, int
     m->W = Nx / 2 + 1;
     m->subtract_mean = subtract_mean != 0;
     size_t spec = 2 * elem(precision) * (size_t)Ny * m->W;
+    hip_check(hipMalloc(&m->rows, spec), "hipMalloc(row spectrum)");
     hip_check(hipMalloc(&m->fmap, spec), "hipMalloc(map spectrum)");
     hip_check(hipMalloc(&m->g0, spec), "hipMalloc(column scratch)");
     hip_check(hipMalloc(&m->g1, spec), "hipMalloc(column scratch)");
@@ -109,6 +110,7 @@ void mlc_map_destroy(mlc_map* m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     if (prev >= 0) (void)hipSetDevice(prev);
   }
+  (void)hipFree(m->rows);
   (void)hipFree(m->fmap);
   (void)hipFree(m->g0);
   (void)hipFree(m->g1);

@@ -12,7 +12,9 @@ struct mlc_map {
   int precision = MLC_F32;
   int Nx = 0, Ny = 0, W = 0;
   int subtract_mean = 0;
-  void* fmap = nullptr;    // Ny x W complex (precision), spectrum / (Nx*Ny)
+  void* rows = nullptr;    // Ny x W complex (precision): R2C of every row of map - mean
+  void* fmap = nullptr;    // Ny x W complex: 2-D spectrum / (Nx*Ny), built on first use
+  bool full_ready = false; // fmap matches rows
   double* mean = nullptr;  // device FP64: map mean (0 when not subtracted)
   void* g0 = nullptr;      // Ny x W complex scratch (value column output)
   void* g1 = nullptr;      // Ny x W complex scratch (derivative column output)
@@ -27,7 +29,7 @@ void fft_check_dims(int Nx, int Ny);
 // n x n profile and d/drhalf, normalised, into device FP64 arrays.
 void fft_profile(mlc_ctx* ctx, const ProfileParams& P, double* prof, double* dprof);
 
-// Map spectrum of d_map into m->fmap (and its FP64 mean into m->mean).
+// Row spectrum of d_map - mean into m->rows (and the FP64 mean into m->mean).
 template <class T>
 void fft_map_transform(mlc_map* m, const double* d_map);
 // Fused convolution (see mlc_map_convolve).

@@ -323,6 +323,85 @@ __global__ __launch_bounds__(kColThreads) void k_node_col(
   }
 }
 
+// Node spectra without column FFTs.  With G the row spectrum of the map
+// (unnormalised R2C of every row) and the kernel factorisation above, the
+// column pass of the 2-D convolution is a short circular filter along y in
+// the row-spectral domain:
+//   H[y][u] = e^{i pi u/Nx}/Nx * sum_{k<h} (a[k][u]/2) (G[y-k][u] + G[y+k+1][u]),
+// value taps av, derivative taps ad.  A workgroup owns kYfCols columns and
+// kYfRows rows; the rows it needs (plus h above and below, circular) are
+// staged in LDS, and every thread keeps kYfOut outputs and two windows of
+// kYfOut + kYfK - 1 inputs in registers over each block of kYfK taps.  The
+// Nyquist column (u = W-1) is zero.  Logical blocks run down each column
+// strip, XCD by XCD, so the halo rows are shared in one L2.
+constexpr int kYfCols = 64, kYfGroups = 8, kYfOut = 8, kYfK = 4;
+constexpr int kYfRows = kYfGroups * kYfOut;
+
+template <class T>
+__global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
+    const cpx<T>* __restrict__ G, const double* __restrict__ av, const double* __restrict__ ad, int h,
+    int hp, const double* __restrict__ phx, cpx<T>* __restrict__ out_v, cpx<T>* __restrict__ out_d,
+    int Nx, int Ny, int W) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem);  // [kYfRows + 2 hp][kYfCols]
+  const int nstrip = (W + kYfCols - 1) / kYfCols, nyb = (Ny + kYfRows - 1) / kYfRows;
+  const int lb = xcd_block(nstrip * nyb);
+  if (lb >= nstrip * nyb) return;
+  const int u0 = (lb / nyb) * kYfCols, y0 = (lb % nyb) * kYfRows;
+  const int c = threadIdx.x % kYfCols, g = threadIdx.x / kYfCols, u = u0 + c;
+  const int nrow = kYfRows + 2 * hp;
+  int yy = (y0 - hp + g) % Ny;  // circular row of tile row r, advanced by kYfGroups
+  if (yy < 0) yy += Ny;
+  for (int r = g; r < nrow; r += kYfGroups) {
+    tile[r * kYfCols + c] = u < W ? G[(size_t)yy * W + u] : mk<T>(0, 0);
+    yy += kYfGroups;
+    while (yy >= Ny) yy -= Ny;  // Ny may be smaller than kYfGroups
+  }
+  __syncthreads();
+  if (u >= W) return;
+  const int ybase = y0 + g * kYfOut;
+  const bool nyq = u == W - 1;
+  cpx<T> accv[kYfOut], accd[kYfOut];
+#pragma unroll
+  for (int i = 0; i < kYfOut; ++i) accv[i] = accd[i] = mk<T>(0, 0);
+  const cpx<T>* col = tile + c;
+  for (int kb = 0; kb < hp; kb += kYfK) {
+    // left[j] = G[ybase - kb - (kYfK-1) + j], right[j] = G[ybase + kb + 1 + j]
+    cpx<T> L[kYfOut + kYfK - 1], R[kYfOut + kYfK - 1];
+    const int lrow = g * kYfOut + hp - kb - (kYfK - 1), rrow = g * kYfOut + hp + kb + 1;
+#pragma unroll
+    for (int j = 0; j < kYfOut + kYfK - 1; ++j) {
+      L[j] = col[(lrow + j) * kYfCols];
+      R[j] = col[(rrow + j) * kYfCols];
+    }
+    T bv[kYfK], bd[kYfK];
+#pragma unroll
+    for (int kk = 0; kk < kYfK; ++kk) {
+      const int k = kb + kk;
+      bv[kk] = k < h ? (T)av[(size_t)k * W + u] : (T)0;
+      bd[kk] = (out_d && k < h) ? (T)ad[(size_t)k * W + u] : (T)0;
+    }
+#pragma unroll
+    for (int kk = 0; kk < kYfK; ++kk)
+#pragma unroll
+      for (int i = 0; i < kYfOut; ++i) {
+        const cpx<T> sm = add(L[i - kk + kYfK - 1], R[i + kk]);
+        accv[i] = mk<T>(accv[i].x + bv[kk] * sm.x, accv[i].y + bv[kk] * sm.y);
+        accd[i] = mk<T>(accd[i].x + bd[kk] * sm.x, accd[i].y + bd[kk] * sm.y);
+      }
+  }
+  const double sc = 0.5 / (double)Nx;
+  const cpx<T> ph = mk<T>((T)(phx[2 * u] * sc), (T)(phx[2 * u + 1] * sc));
+#pragma unroll
+  for (int i = 0; i < kYfOut; ++i) {
+    const int y = ybase + i;
+    if (y < Ny) {
+      out_v[(size_t)y * W + u] = nyq ? mk<T>(0, 0) : mul(accv[i], ph);
+      if (out_d) out_d[(size_t)y * W + u] = nyq ? mk<T>(0, 0) : mul(accd[i], ph);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ profiles
 // One workgroup evaluates the n x n profile and its d/drhalf, normalises both
 // (K = f/S, dK = (df - K dS)/S) in FP64.
@@ -642,6 +721,22 @@ void node_col(mlc_ctx* ctx, const cpx<T>* fmap, const double* av, const double* 
   });
 }
 
+// y-filter node spectra from the row spectrum; false when the staged rows
+// do not fit in LDS (the caller then takes the column-FFT path).
+template <class T>
+bool node_yfilter(mlc_ctx* ctx, const cpx<T>* rows, const double* av, const double* ad, int h,
+                  cpx<T>* ov, cpx<T>* od, int Nx, int Ny, int W) {
+  const int hp = (h + kYfK - 1) / kYfK * kYfK;
+  const size_t lds = sizeof(cpx<T>) * (size_t)(kYfRows + 2 * hp) * kYfCols;
+  if (lds > kLdsBytes) return false;
+  const int nb = ((W + kYfCols - 1) / kYfCols) * ((Ny + kYfRows - 1) / kYfRows);
+  auto k = k_node_yfilter<T>;
+  allow_lds(k, lds);
+  MLC_LAUNCH(ctx, "conv_node_yfilter", k, dim3(((nb + 7) / 8) * 8), dim3(kYfCols * kYfGroups), lds,
+             rows, av, ad, h, hp, cos_table(ctx, 3, Nx), ov, od, Nx, Ny, W);
+  return true;
+}
+
 }  // namespace
 
 namespace mlc {
@@ -662,9 +757,21 @@ void fft_map_transform(mlc_map* m, const double* d_map) {
   } else {
     hip_check(hipMemsetAsync(m->mean, 0, sizeof(double), ctx->stream), "hipMemsetAsync(mean)");
   }
-  cpx<T>* F = (cpx<T>*)m->fmap;
-  row_r2c<T, double>(ctx, d_map, F, m->Nx, m->Ny, m->mean);
-  col_fft<T, false>(ctx, F, m->Ny, m->W, (T)(1.0 / (double)npix));
+  row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, m->mean);
+  m->full_ready = false;
+}
+
+// The full 2-D spectrum / (Nx Ny) in m->fmap, from the row spectrum, on
+// first use after a map update (the plain path and the column-FFT node path).
+template <class T>
+void ensure_full(mlc_map* m) {
+  if (m->full_ready) return;
+  const size_t bytes = sizeof(cpx<T>) * (size_t)m->Ny * m->W;
+  hip_check(hipMemcpyAsync(m->fmap, m->rows, bytes, hipMemcpyDeviceToDevice, m->ctx->stream),
+            "hipMemcpyAsync(row spectrum)");
+  col_fft<T, false>(m->ctx, (cpx<T>*)m->fmap, m->Ny, m->W,
+                    (T)(1.0 / ((double)m->Nx * (double)m->Ny)));
+  m->full_ready = true;
 }
 
 template <class T>
@@ -684,7 +791,11 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
              m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr);
   cpx<T>* G0 = (cpx<T>*)m->g0;
   cpx<T>* G1 = (cpx<T>*)m->g1;
-  node_col<T>(ctx, (const cpx<T>*)m->fmap, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny, W);
+  if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny,
+                       W)) {
+    ensure_full<T>(m);
+    node_col<T>(ctx, (const cpx<T>*)m->fmap, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny, W);
+  }
   if (d_node) row_c2r<T, 2>(ctx, G0, G1, (T*)d_node, m->Nx, m->Ny, m->mean);
   if (d_value) row_c2r<T, 1>(ctx, G0, nullptr, (T*)d_value, m->Nx, m->Ny, m->mean);
 }
@@ -706,6 +817,7 @@ void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void*
   cpx<T>* fk = ws.get<cpx<T>>(ifk);
   T* outv = d_value ? (T*)d_value : ws.get<T>(iv);
   T* outd = ws.get<T>(id);
+  ensure_full<T>(m);
   MLC_LAUNCH(ctx, "profile", k_profile, dim3(1), dim3(1024), 0, P, prof, dprof);
   sync(ctx);
   const int grid = std::min<int64_t>(cdiv(npix, 256), fill_grid(ctx, 16));

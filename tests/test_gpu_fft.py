@@ -120,3 +120,22 @@ def test_unsupported_sizes_rejected(ctx):
     spec = api.MapSpectrum(ctx, torch.ones((32, 32), dtype=torch.float64, device="cuda"), "f32")
     with pytest.raises(Exception, match="larger than the map"):
         spec.convolve("gaussian", 40.0)
+
+
+@pytest.mark.parametrize("Ny,Nx,shape,rhalf", [(36, 40, "gaussian", 8.0),     # rows wrap, partial row block
+                                               (256, 256, "uniform", 50.0),   # FP64 falls back to column FFTs
+                                               (100, 120, "exponential", 3.1)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec):
+    rng = np.random.default_rng(Ny + Nx)
+    m = _smooth_map(rng, Ny, Nx)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
+    val, node = spec.convolve(shape, rhalf, 1.0, value=True, node=True)
+    prof, dprof = orc.profile(shape, rhalf, 1.0, deriv=True)
+    ref = orc.fft_convolve(m, orc.set_kernel(prof, Nx, Ny))
+    dref = orc.fft_convolve(m, orc.set_kernel(dprof, Nx, Ny))
+    tol = 1e-12 if prec == "f64" else 2e-6
+    assert _rel(val.cpu().numpy().astype(np.float64), ref) < tol
+    nd = node.cpu().numpy().astype(np.float64)
+    assert _rel(nd[..., 0], ref) < tol
+    assert np.max(np.abs(nd[..., 1] - dref)) < tol * 20 * max(np.max(np.abs(ref)), 1.0)
