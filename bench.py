@@ -78,18 +78,36 @@ def bilinear(img, x, y):
             + fy * ((1 - fx) * img[iy + 1, ix] + fx * img[iy + 1, ix + 1]))
 
 
-def algorithmic_bytes(name, esz, ncurves, nsamp):
-    """HBM bytes one launch must move at least (value bytes of its operands)."""
+def algorithmic_bytes(name, esz, ncurves):
+    """Bytes one launch must move at least (value bytes of its operands).
+
+    A chi2 launch covers one (profile, band): the curves of that profile,
+    one band's epochs each; every sample reads 4 bilinear taps of (mu,
+    dmu/dr) (8 esz bytes), every curve its 64-byte record and its 8 FP64
+    accumulators (read and written)."""
     W = N // 2 + 1
     spec = N * W * 2 * esz          # one half spectrum
     real = N * N * esz              # one real map
+    curves = ncurves // len(SHAPES)
     return {
         "conv_node_col": spec + 2 * spec,              # read map spectrum, write 2 spectra
+        "conv_node_yfilter": spec + 2 * spec,          # read row spectrum, write 2 spectra
         "fft_row_c2r_node": 2 * spec + 2 * real,       # read 2 spectra, write (mu, dmu/dr)
         "fft_row_r2c": N * N * 8 + spec,               # read FP64 map, write spectrum
-        "fft_col_fwd": 2 * spec,
-        "lc_chi2_grad": ncurves * (7 * 8 + 8 + 8 * 8) + nsamp * 4 * 2 * esz,
+        "lc_chi2_grad": curves * EPOCHS * 4 * 2 * esz + curves * (64 + 2 * 64),
     }.get(name)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a device copy), or
+    None when it holds no figure for the kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_4096_f32.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def run_step(ctx, ws, plain=False):
@@ -185,6 +203,29 @@ def cpu_baseline(m, params, prof, times, mobs, sig, nodes64_host, budget_s=20.0)
                        f"FFTs + {ncurves} curves + sort ({t_step:.1f} s)")}
 
 
+def pmc_step():
+    """Two FP32 steps of the workload, then a 512 MiB device-to-device copy
+    whose FETCH_SIZE / WRITE_SIZE calibrate the byte counters."""
+    import torch
+
+    import microsn_amd.api as api
+
+    ctx = api.Context(0)
+    m_dev = torch.tensor(make_map(), device="cuda")
+    params, prof, times, _, _ = make_inputs()
+    mobs = [np.full(len(t), -1.7) for t in times]
+    sig = [np.full(len(t), 0.05) for t in times]
+    ws = setup(ctx, "f32", m_dev, params, prof, times, mobs, sig)
+    for _ in range(2):
+        run_step(ctx, ws)
+    torch.cuda.synchronize()
+    a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.ones_like(a)
+    a.copy_(b)
+    torch.cuda.synchronize()
+    print("pmc step done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +233,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain leg")
+    ap.add_argument("--pmc-step", action="store_true",
+                    help="run two FP32 steps and a 512 MiB device copy (the byte-counter "
+                         "calibration) for rocprofv3 --pmc, print nothing else")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -203,6 +247,9 @@ def main():
     import microsn_amd.api as api
     import microsn_amd.lightcurves as lcm
 
+    if args.pmc_step:
+        pmc_step()
+        return
     t_setup = time.perf_counter()
     m = make_map()
     params, prof, times, truth, rng = make_inputs()
@@ -220,7 +267,7 @@ def main():
         mobs.append(-2.5 * np.log10(bilinear(v, x, y)) + rng.normal(0.0, 0.05, t.size))
         sig.append(np.full(t.size, 0.05))
     s64.close()
-    ncurves, nsamp = len(prof), len(prof) * EPOCHS * len(BAND_SCALE)
+    ncurves = len(prof)
 
     # ---- FP32 fused path (the metric)
     ws32 = setup(ctx, "f32", m_dev, params, prof, times, mobs, sig)
@@ -253,12 +300,12 @@ def main():
         torch.cuda.empty_cache()
 
     # ---- roofline of the dominant kernel
-    dom = max((k for k in kern if algorithmic_bytes(k, 4, ncurves, nsamp)), key=lambda k: kern[k][0])
+    dom = max((k for k in kern if algorithmic_bytes(k, 4, ncurves)), key=lambda k: kern[k][0])
     ms_tot, launches = kern[dom]
     avg_s = ms_tot / launches / 1e3
-    ab = algorithmic_bytes(dom, 4, ncurves, nsamp)
+    ab = algorithmic_bytes(dom, 4, ncurves)
     roof = {"kernel": dom, "bound": "hbm", "achieved": ab / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": None,
+            "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": measured_traffic(dom),
             "algorithmic_bytes_per_launch": ab, "avg_launch_ms": avg_s * 1e3}
 
     cpu = None
