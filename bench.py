@@ -272,7 +272,11 @@ def main():
     # ---- FP32 fused path (the metric)
     ws32 = setup(ctx, "f32", m_dev, params, prof, times, mobs, sig)
     t_setup = time.perf_counter() - t_setup
-    dt32, out32, kern = time_steps(ctx, ws32, args.steps, args.warmup, plain=False, timing=True)
+    dt32, out32, _ = time_steps(ctx, ws32, args.steps, args.warmup, plain=False, timing=False)
+    # the same steps again with a HIP event pair around every launch: the
+    # per-kernel breakdown and the roofline's launch durations
+    ksteps = max(1, min(args.steps, 5))
+    dt32_ev, _, kern = time_steps(ctx, ws32, ksteps, 0, plain=False, timing=True)
     chi32, grad32, lo32, npb32, be32 = out32
 
     # ---- FP64 plain path (same workload)
@@ -328,7 +332,8 @@ def main():
                    "nodes": len(SHAPES) * len(BAND_SCALE)},
         "fp64_plain": plain, "speedup_vs_fp64_plain": (ncurves / dt32) / plain["value"] if plain else None,
         "precision": err, "roofline": roof, "cpu_baseline": cpu,
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kern.items())},
+        "kernels_ms_per_step": {k: v[0] / ksteps for k, v in sorted(kern.items())},
+        "ms_per_step_with_kernel_events": dt32_ev * 1e3,
         "setup_s": t_setup,
     }
     print(json.dumps(line))

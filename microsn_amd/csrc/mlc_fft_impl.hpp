@@ -29,6 +29,7 @@
 
 #include "mlc_fft_core.hpp"
 #include "mlc_internal.hpp"
+#include "mlc_ctfft.hpp"
 #include "mlc_fft.hpp"
 #include "mlc_profile.hpp"
 #include "mlc_tiles.hpp"
@@ -352,10 +353,19 @@ __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
   const int nrow = kYfRows + 2 * hp;
   int yy = (y0 - hp + g) % Ny;  // circular row of tile row r, advanced by kYfGroups
   if (yy < 0) yy += Ny;
-  for (int r = g; r < nrow; r += kYfGroups) {
-    tile[r * kYfCols + c] = u < W ? G[(size_t)yy * W + u] : mk<T>(0, 0);
-    yy += kYfGroups;
-    while (yy >= Ny) yy -= Ny;  // Ny may be smaller than kYfGroups
+  const bool col_in = u < W;
+  for (int r0 = g; r0 < nrow; r0 += 4 * kYfGroups) {
+    // four rows per thread in flight before the LDS stores
+    cpx<T> v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v[e] = (col_in && r0 + e * kYfGroups < nrow) ? G[(size_t)yy * W + u] : mk<T>(0, 0);
+      yy += kYfGroups;
+      while (yy >= Ny) yy -= Ny;  // Ny may be smaller than kYfGroups
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (r0 + e * kYfGroups < nrow) tile[(r0 + e * kYfGroups) * kYfCols + c] = v[e];
   }
   __syncthreads();
   if (u >= W) return;
@@ -399,6 +409,67 @@ __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
       out_v[(size_t)y * W + u] = nyq ? mk<T>(0, 0) : mul(accv[i], ph);
       if (out_d) out_d[(size_t)y * W + u] = nyq ? mk<T>(0, 0) : mul(accd[i], ph);
     }
+  }
+}
+
+// Node-map rows from the two node spectra by one length-N complex inverse
+// FFT per row (compile-time N, mlc_ctfft.hpp): with
+//   Z[u] = Hv[u] + i Hd[u],  Z[N-u] = conj(Hv[u]) + i conj(Hd[u]),  0 < u < N/2
+// (DC and Nyquist keep only their real parts, as a C2R does), z[x] = v[x] +
+// i d[x] is the (value, d/dr) pair of pixel x.  The value gets the map mean
+// back.  Rows run XCD by XCD so the four rows of a 4 x 4 tile meet in one L2.
+template <class T, int N>
+__global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
+    const cpx<T>* __restrict__ Hv, const cpx<T>* __restrict__ Hd, cpx<T>* __restrict__ out, int Ny,
+    const double* __restrict__ shift, const cpx<T>* __restrict__ tw) {
+  using Gm = CtGeom<N>;
+  constexpr int G = Gm::G, W = N / 2 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* hi = lo + Gm::NLO;
+  cpx<T>* buf = hi + Gm::NHI;
+  const int y = xcd_block(Ny);
+  if (y >= Ny) return;
+  const int t = threadIdx.x;
+  for (int i = t; i < Gm::NLO + Gm::NHI; i += G) lo[i] = tw[i];
+  const cpx<T>* hv = Hv + (size_t)y * W;
+  const cpx<T>* hd = Hd + (size_t)y * W;
+  {
+    // u = t + i G, i < 8, covers [0, N/2); all loads issued before the stores
+    constexpr int NU = (N / 2) / G;
+    cpx<T> a[NU], b[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+      a[i] = hv[t + i * G];
+      b[i] = hd[t + i * G];
+    }
+    cpx<T> an, bn;  // Nyquist
+    if (t == 0) {
+      an = hv[N / 2];
+      bn = hd[N / 2];
+    }
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+      const int u = t + i * G;
+      if (u == 0) {
+        buf[pad16(0)] = mk<T>(a[i].x, b[i].x);
+      } else {
+        buf[pad16(u)] = mk<T>(a[i].x - b[i].y, a[i].y + b[i].x);
+        buf[pad16(N - u)] = mk<T>(a[i].x + b[i].y, b[i].x - a[i].y);
+      }
+    }
+    if (t == 0) buf[pad16(N / 2)] = mk<T>(an.x, bn.x);
+  }
+  __syncthreads();
+  ct_fft<T, N, true>(buf, lo, hi, t);
+  const T sh = shift ? (T)*shift : (T)0;
+  // pixel x = t + j G: consecutive lanes read consecutive LDS elements and
+  // four of them fill one 32-byte row of a tile
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int x = t + j * G;
+    const cpx<T> z = buf[pad16(x)];
+    out[tile_offset(x, y, N)] = mk<T>(z.x + sh, z.y);
   }
 }
 
@@ -721,6 +792,62 @@ void node_col(mlc_ctx* ctx, const cpx<T>* fmap, const double* av, const double* 
   });
 }
 
+// Inverse twiddle tables of mlc_ctfft.hpp for length N: lo[l] = w^l,
+// l < 2^LB, then hi[h] = w^(h 2^LB), w = exp(2 pi i/N); cached per context.
+template <class T, int N>
+const cpx<T>* ct_twiddles(mlc_ctx* ctx) {
+  using Gm = CtGeom<N>;
+  auto key = std::make_pair(N, 3000 + (int)sizeof(T));
+  auto it = ctx->twiddles.find(key);
+  if (it != ctx->twiddles.end()) return (const cpx<T>*)it->second;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  std::vector<cpx<T>> h;
+  for (int l = 0; l < Gm::NLO; ++l)
+    h.push_back(mk<T>((T)std::cos(2 * pi * l / N), (T)std::sin(2 * pi * l / N)));
+  for (int k = 0; k < Gm::NHI; ++k) {
+    const long double a = 2 * pi * (long double)k * Gm::NLO / N;
+    h.push_back(mk<T>((T)std::cos(a), (T)std::sin(a)));
+  }
+  void* d = nullptr;
+  hip_check(hipMalloc(&d, sizeof(cpx<T>) * h.size()), "hipMalloc(twiddles)");
+  ctx->twiddles[key] = d;
+  hip_check(hipMemcpy(d, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice),
+            "hipMemcpy(twiddles)");
+  return (const cpx<T>*)d;
+}
+
+// Node maps from the node spectra with the compile-time row FFT; false for
+// widths it does not cover (not a power of two in [256, 16384], or rows
+// that do not fit in LDS).
+template <class T>
+bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx, int Ny,
+               const double* shift) {
+  bool done = false;
+  auto go = [&](auto nc) {
+    constexpr int N = decltype(nc)::value;
+    using Gm = CtGeom<N>;
+    const size_t lds = sizeof(cpx<T>) * (size_t)(Gm::NLO + Gm::NHI + Gm::LDS);
+    if (lds > kLdsBytes) return;
+    auto k = k_node_rows<T, N>;
+    allow_lds(k, lds);
+    MLC_LAUNCH(ctx, "fft_row_c2r_node", k, dim3(((Ny + 7) / 8) * 8), dim3(Gm::G), lds, Hv, Hd,
+               (cpx<T>*)out, Ny, shift, ct_twiddles<T, N>(ctx));
+    done = true;
+  };
+  using std::integral_constant;
+  switch (Nx) {
+    case 256: go(integral_constant<int, 256>{}); break;
+    case 512: go(integral_constant<int, 512>{}); break;
+    case 1024: go(integral_constant<int, 1024>{}); break;
+    case 2048: go(integral_constant<int, 2048>{}); break;
+    case 4096: go(integral_constant<int, 4096>{}); break;
+    case 8192: go(integral_constant<int, 8192>{}); break;
+    case 16384: go(integral_constant<int, 16384>{}); break;
+    default: break;
+  }
+  return done;
+}
+
 // y-filter node spectra from the row spectrum; false when the staged rows
 // do not fit in LDS (the caller then takes the column-FFT path).
 template <class T>
@@ -796,7 +923,8 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
     ensure_full<T>(m);
     node_col<T>(ctx, (const cpx<T>*)m->fmap, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny, W);
   }
-  if (d_node) row_c2r<T, 2>(ctx, G0, G1, (T*)d_node, m->Nx, m->Ny, m->mean);
+  if (d_node && !node_rows<T>(ctx, G0, G1, (T*)d_node, m->Nx, m->Ny, m->mean))
+    row_c2r<T, 2>(ctx, G0, G1, (T*)d_node, m->Nx, m->Ny, m->mean);
   if (d_value) row_c2r<T, 1>(ctx, G0, nullptr, (T*)d_value, m->Nx, m->Ny, m->mean);
 }
 

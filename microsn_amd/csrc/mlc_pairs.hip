@@ -80,8 +80,10 @@ __global__ __launch_bounds__(256) void k_chi2_pairs(const double* __restrict__ m
 }
 
 // Per-bin sums over contiguous chunks (<= kChunk elements, never crossing a
-// bin).  part[2*c] = sum exp(-0.5 chi2), part[2*c+1] = sum chi2.
-constexpr int kChunk = 1 << 16;
+// bin), one workgroup per chunk; the chunk sums of a bin are added on the
+// host in chunk order, so the result does not depend on scheduling.
+// part[2*c] = sum exp(-0.5 chi2), part[2*c+1] = sum chi2.
+constexpr int kChunk = 1 << 12;
 
 __global__ __launch_bounds__(256) void k_bin_chunks(const double* __restrict__ chi2,
                                                     const unsigned long long* __restrict__ cstart,

@@ -8,11 +8,14 @@
 // -0.0 becomes +0.0 (they compare equal), every NaN becomes the largest
 // value (NaN sorts after +inf; among NaNs the original index order is kept),
 // negative values are bit-inverted and positive ones get the sign bit set.
-// Eight 8-bit digit passes; a pass whose digit is the same for every key is
-// skipped (the all-digit histogram is taken in the transform kernel).
+// Eight 8-bit digit passes.  The transform kernel histograms all eight
+// digits; a pass whose digit is the same for every key is trivial and its
+// kernels reduce to a copy.  The decision is taken on the device by every
+// kernel of the pass, so the sort never waits for the host.
 //
-// One pass over a tile of 4096 keys (256 threads x 16 rounds, round r holds
-// keys r*256 + t, i.e. tile order = (round, thread)):
+// One pass over tiles of 256*R keys (256 threads x R rounds, round r holds
+// keys r*256 + t, i.e. tile order = (round, thread); R = 4 for small sorts,
+// 16 for large ones):
 //   k_tile_hist  per-tile digit counts, digit-major [256][ntiles]
 //   k_scan_tiles exclusive offsets: digit base + earlier tiles of that digit
 //   k_scatter    stable rank inside the tile (wavefront match on the 8 digit
@@ -28,9 +31,13 @@ using namespace mlc;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRounds = 16;
-constexpr int kTile = kThreads * kRounds;  // 4096
 constexpr int kWaves = kThreads / 64;
+constexpr int64_t kSmallSort = 1 << 21;  // up to this many keys: 1024-key tiles
+
+// True when every key of the pass has the same digit (block-uniform).
+__device__ __forceinline__ bool pass_trivial(const unsigned* __restrict__ hist_pass, int64_t n) {
+  return __syncthreads_or(hist_pass[threadIdx.x] == (unsigned)n) != 0;
+}
 
 __device__ __forceinline__ uint64_t key_of(double x) {
   uint64_t b = (uint64_t)__double_as_longlong(x);
@@ -64,15 +71,18 @@ __global__ __launch_bounds__(kThreads) void k_transform_hist(const double* __res
   }
 }
 
+template <int R>
 __global__ __launch_bounds__(kThreads) void k_tile_hist(const uint64_t* __restrict__ keys, int64_t n,
                                                         int pass, int ntiles,
-                                                        unsigned* __restrict__ counts) {
+                                                        unsigned* __restrict__ counts,
+                                                        const unsigned* __restrict__ hist_pass) {
+  if (pass_trivial(hist_pass, n)) return;
   __shared__ unsigned h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t base = (int64_t)blockIdx.x * (kThreads * R);
 #pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
+  for (int r = 0; r < R; ++r) {
     int64_t i = base + r * kThreads + threadIdx.x;
     if (i < n) atomicAdd(&h[digit_of(keys[i], pass)], 1u);
   }
@@ -80,47 +90,69 @@ __global__ __launch_bounds__(kThreads) void k_tile_hist(const uint64_t* __restri
   counts[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// One workgroup per digit: exclusive scan over the tiles, plus the digit's
-// global base (sum of the counts of all smaller digits for this pass).
-__global__ __launch_bounds__(1024) void k_scan_tiles(unsigned* __restrict__ counts, int ntiles,
-                                                     const unsigned* __restrict__ hist_pass) {
-  __shared__ unsigned s[1024];
-  __shared__ unsigned carry;
+// One workgroup per digit d: exclusive scan of the digit's tile counts
+// (each thread sums a contiguous run of tiles, the 256 run totals are
+// scanned in LDS) plus the digit's base, the number of keys with a smaller
+// digit (a block reduction over the pass histogram).
+__global__ __launch_bounds__(kThreads) void k_scan_tiles(unsigned* __restrict__ counts, int ntiles,
+                                                         const unsigned* __restrict__ hist_pass,
+                                                         int64_t n) {
+  if (pass_trivial(hist_pass, n)) return;
+  __shared__ unsigned s[kThreads];
   const int d = blockIdx.x, t = threadIdx.x;
-  if (t == 0) {
-    unsigned b = 0;
-    for (int j = 0; j < d; ++j) b += hist_pass[j];
-    carry = b;
+  // digit base
+  s[t] = t < d ? hist_pass[t] : 0u;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (t < w) s[t] += s[t + w];
+    __syncthreads();
   }
+  const unsigned base = s[0];
   __syncthreads();
   unsigned* row = counts + (size_t)d * ntiles;
-  for (int start = 0; start < ntiles; start += 1024) {
-    int i = start + t;
-    unsigned v = i < ntiles ? row[i] : 0u;
-    s[t] = v;
+  const int seg = (ntiles + kThreads - 1) / kThreads;
+  const int a = min(ntiles, t * seg), e = min(ntiles, a + seg);
+  unsigned sum = 0;
+  for (int i = a; i < e; ++i) sum += row[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {  // inclusive Hillis-Steele
+    const unsigned v = t >= off ? s[t - off] : 0u;
     __syncthreads();
-    // Hillis-Steele inclusive scan
-    for (int off = 1; off < 1024; off <<= 1) {
-      unsigned a = t >= off ? s[t - off] : 0u;
-      __syncthreads();
-      s[t] += a;
-      __syncthreads();
-    }
-    unsigned c = carry;
-    if (i < ntiles) row[i] = c + s[t] - v;
+    s[t] += v;
     __syncthreads();
-    if (t == 1023) carry = c + s[1023];
-    __syncthreads();
+  }
+  unsigned run = base + s[t] - sum;
+  for (int i = a; i < e; ++i) {
+    const unsigned v = row[i];
+    row[i] = run;
+    run += v;
   }
 }
 
-// Stable scatter of one pass.  payload_in == nullptr means the identity.
+// Stable scatter of one pass.  payload_in == nullptr means the identity.  A
+// trivial pass copies keys and payload unchanged.
+template <int R>
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict__ kin,
                                                       const unsigned* __restrict__ pin,
                                                       uint64_t* __restrict__ kout,
                                                       unsigned* __restrict__ pout, int64_t n,
                                                       int pass, int ntiles,
-                                                      const unsigned* __restrict__ offsets) {
+                                                      const unsigned* __restrict__ offsets,
+                                                      const unsigned* __restrict__ hist_pass) {
+  constexpr int kRounds = R, kTile = kThreads * R;
+  if (pass_trivial(hist_pass, n)) {
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = base + r * kThreads + threadIdx.x;
+      if (i < n) {
+        kout[i] = kin[i];
+        pout[i] = pin ? pin[i] : (unsigned)i;
+      }
+    }
+    return;
+  }
   __shared__ uint64_t sk[kTile];
   __shared__ unsigned sp[kTile];
   __shared__ unsigned wave_cnt[kWaves][256];
@@ -212,12 +244,6 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
   }
 }
 
-__global__ void k_iota(unsigned* __restrict__ p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = (unsigned)i;
-}
-
 __global__ void k_rank_and_gather(const unsigned* __restrict__ perm, const double* __restrict__ keys,
                                   int64_t n, unsigned* __restrict__ rank, double* __restrict__ sorted) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -253,16 +279,45 @@ __global__ void k_upper_bound(const double* __restrict__ s, int64_t n, const dou
   upper[i] = (unsigned)lo;
 }
 
+template <int R>
+void sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* pbuf, unsigned* perm,
+                 unsigned* cnt, const unsigned* hist, int64_t n) {
+  constexpr int kTile = kThreads * R;
+  const int ntiles = (int)((n + kTile - 1) / kTile);
+  // eight passes (trivial ones copy): ping-pong so that the last one writes perm
+  unsigned* pa = pbuf;  // payload written by passes 0, 2, 4, 6
+  unsigned* pb = perm;  // payload written by passes 1, 3, 5, 7
+  const uint64_t* kin = k0;
+  uint64_t* kout = k1;
+  const unsigned* pin = nullptr;
+  for (int p = 0; p < 8; ++p) {
+    unsigned* pout = (p % 2 == 0) ? pa : pb;
+    const unsigned* hp = hist + p * 256;
+    MLC_LAUNCH(ctx, "sort_tile_hist", k_tile_hist<R>, dim3(ntiles), dim3(kThreads), 0, kin, n, p,
+               ntiles, cnt, hp);
+    MLC_LAUNCH(ctx, "sort_scan_tiles", k_scan_tiles, dim3(256), dim3(kThreads), 0, cnt, ntiles, hp, n);
+    MLC_LAUNCH(ctx, "sort_scatter", k_scatter<R>, dim3(ntiles), dim3(kThreads), 0, kin, pin, kout,
+               pout, n, p, ntiles, cnt, hp);
+    const uint64_t* nk = kout;
+    kout = (kout == k1) ? k0 : k1;
+    kin = nk;
+    pin = pout;
+  }
+}
+
 }  // namespace
 
 namespace mlc {
 
-// Stable argsort into d_perm (sorted position -> original index).
+// Stable argsort into d_perm (sorted position -> original index).  Never
+// synchronises with the host.
 void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsigned* rank,
               double* sorted) {
   if (n <= 0) return;
   MLC_REQUIRE(n < ((int64_t)1 << 32), "sort: more than 2^32-1 keys");
-  const int ntiles = (int)((n + kTile - 1) / kTile);
+  const bool small = n <= kSmallSort;
+  const int64_t tile = kThreads * (small ? 4 : 16);
+  const int64_t ntiles = (n + tile - 1) / tile;
   Workspace ws(ctx);
   int ik0 = ws.add(sizeof(uint64_t) * n), ik1 = ws.add(sizeof(uint64_t) * n);
   int ip1 = ws.add(sizeof(unsigned) * n);
@@ -272,47 +327,11 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   uint64_t *k0 = ws.get<uint64_t>(ik0), *k1 = ws.get<uint64_t>(ik1);
   unsigned *pbuf = ws.get<unsigned>(ip1), *hist = ws.get<unsigned>(ih), *cnt = ws.get<unsigned>(ic);
   hip_check(hipMemsetAsync(hist, 0, sizeof(unsigned) * 8 * 256, ctx->stream), "hipMemsetAsync(hist)");
-  const int g = std::min<int64_t>(cdiv(n, kThreads), fill_grid(ctx, 4));
+  const int g = std::min<int64_t>(ntiles, fill_grid(ctx, 1));
   MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
              hist);
-  std::vector<unsigned> h(8 * 256);
-  d2h(ctx, h.data(), hist, sizeof(unsigned) * h.size());
-  sync(ctx);
-  std::vector<int> passes;
-  for (int p = 0; p < 8; ++p) {
-    bool trivial = false;
-    for (int d = 0; d < 256; ++d)
-      if (h[p * 256 + d] == (unsigned)n) trivial = true;
-    if (!trivial) passes.push_back(p);
-  }
-  if (passes.empty()) {
-    MLC_LAUNCH(ctx, "sort_iota", k_iota, dim3(std::min<int64_t>(cdiv(n, 256), 4096)), dim3(256), 0,
-               perm, n);
-  } else {
-    // ping-pong so that the last pass writes its payload into perm
-    const int np = (int)passes.size();
-    unsigned* pa = (np % 2 == 1) ? perm : pbuf;  // output of pass 0
-    unsigned* pb = (np % 2 == 1) ? pbuf : perm;
-    const uint64_t* kin = k0;
-    uint64_t* kout = k1;
-    const unsigned* pin = nullptr;
-    unsigned* pout = pa;
-    for (int q = 0; q < np; ++q) {
-      const int p = passes[q];
-      MLC_LAUNCH(ctx, "sort_tile_hist", k_tile_hist, dim3(ntiles), dim3(kThreads), 0, kin, n, p,
-                 ntiles, cnt);
-      MLC_LAUNCH(ctx, "sort_scan_tiles", k_scan_tiles, dim3(256), dim3(1024), 0, cnt, ntiles,
-                 hist + p * 256);
-      MLC_LAUNCH(ctx, "sort_scatter", k_scatter, dim3(ntiles), dim3(kThreads), 0, kin, pin, kout,
-                 pout, n, p, ntiles, cnt);
-      // next pass reads what this one wrote
-      const uint64_t* nk = kout;
-      kout = (kout == k1) ? k0 : k1;
-      kin = nk;
-      pin = pout;
-      pout = (pout == pa) ? pb : pa;
-    }
-  }
+  if (small) sort_passes<4>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
+  else sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
   if (rank || sorted)
     MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
                dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, keys,

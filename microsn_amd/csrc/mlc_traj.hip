@@ -139,7 +139,7 @@ struct alignas(64) CurveRec {
 static_assert(sizeof(CurveRec) == 64, "one record per 64 bytes");
 
 constexpr int kPartChunk = 2048;    // curves per partition block (256 threads x 8)
-constexpr int kMaxProfiles = 4096;  // bucket counters live in LDS
+constexpr int kMaxBuckets = 8192;   // partition bucket counters live in LDS
 constexpr int kMaxBandNodes = 16;   // radius nodes of one (profile, band), kernel argument
 #ifndef MLC_LC_LANES
 #define MLC_LC_LANES 16
@@ -147,31 +147,79 @@ constexpr int kMaxBandNodes = 16;   // radius nodes of one (profile, band), kern
 constexpr int kLanes = MLC_LC_LANES;  // lanes per curve in a band launch
 static_assert(kLanes >= 8 && kLanes <= 64 && (kLanes & (kLanes - 1)) == 0, "lanes per curve");
 
-// Per-block bucket counts, stored bucket-major: bh[q * nblk + block].
-__global__ __launch_bounds__(256) void k_part_count(const int* __restrict__ prof, int n, int P,
-                                                    int* __restrict__ bh) {
-  extern __shared__ int h[];
-  for (int q = threadIdx.x; q <= P; q += 256) h[q] = 0;
-  __syncthreads();
-  const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
-  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
-    const int p = prof[c];
-    atomicAdd(&h[(p >= 0 && p < P) ? p : P], 1);
+// Buckets of the partition: profile p and a coarse map cell of the curve's
+// position at the middle of its epochs, cells in Morton order, so that
+// curves processed together read nearby parts of the node maps; bucket P*C
+// collects curves with an invalid profile index.
+struct PartGeom {
+  int P, C, L;       // profiles, cells (4^L), bits per cell axis
+  double sx, sy;     // cells per pixel along x and y
+  const int* prof;
+  const double* params;
+  const int* obs;
+  int nobs, B, nepoch;
+  const int* obs_off;
+  const double* tt;
+};
+
+__device__ __forceinline__ int curve_bucket(const PartGeom& g, int c) {
+  const int p = g.prof[c];
+  if (!(p >= 0 && p < g.P)) return g.P * g.C;
+  double tm = 0.0;
+  const int o = g.obs[c];
+  if (o >= 0 && o < g.nobs) {
+    const int e0 = g.obs_off[(size_t)o * g.B], e1 = g.obs_off[(size_t)o * g.B + g.B];
+    if (e0 >= 0 && e1 <= g.nepoch && e0 < e1) tm = 0.5 * (g.tt[e0] + g.tt[e1 - 1]);
   }
-  __syncthreads();
-  for (int q = threadIdx.x; q <= P; q += 256) bh[(size_t)q * gridDim.x + blockIdx.x] = h[q];
+  const double* Pc = g.params + (size_t)c * 7;
+  double sn, cs;
+  sincos(Pc[2], &sn, &cs);
+  const double x = (Pc[0] + Pc[3] * cs * tm) * g.sx, y = (Pc[1] + Pc[3] * sn * tm) * g.sy;
+  const int side = 1 << g.L;
+  const int cx = (x >= 0.0) ? (int)min(x, (double)(side - 1)) : 0;  // NaN -> 0
+  const int cy = (y >= 0.0) ? (int)min(y, (double)(side - 1)) : 0;
+  int m = 0;
+  for (int bit = 0; bit < g.L; ++bit) m |= (((cx >> bit) & 1) << (2 * bit)) | (((cy >> bit) & 1) << (2 * bit + 1));
+  return p * g.C + m;
 }
 
-// Exclusive scan of the m = (P+1)*nblk block counts in bucket-major order (one
-// workgroup), then start[q] / cnt[q] of every bucket.
-__global__ __launch_bounds__(1024) void k_part_scan(int* __restrict__ bh, int m, int nblk, int P, int n,
+// Per-block bucket counts, stored block-major: bh[block * nq + q].
+__global__ __launch_bounds__(256) void k_part_count(PartGeom g, int n, int* __restrict__ bh) {
+  extern __shared__ int h[];
+  const int nq = g.P * g.C + 1;
+  for (int q = threadIdx.x; q < nq; q += 256) h[q] = 0;
+  __syncthreads();
+  const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
+  for (int c = c0 + threadIdx.x; c < c1; c += 256) atomicAdd(&h[curve_bucket(g, c)], 1);
+  __syncthreads();
+  for (int q = threadIdx.x; q < nq; q += 256) bh[(size_t)blockIdx.x * nq + q] = h[q];
+}
+
+// Per bucket q (one thread each): exclusive prefix over the blocks, in
+// place, and the bucket total.
+__global__ __launch_bounds__(256) void k_part_colscan(int* __restrict__ bh, int nq, int nblk,
+                                                      int* __restrict__ tot) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nq) return;
+  int run = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int v = bh[(size_t)b * nq + q];
+    bh[(size_t)b * nq + q] = run;
+    run += v;
+  }
+  tot[q] = run;
+}
+
+// Exclusive scan of the nq bucket totals (one workgroup, nq <= 8192) into
+// base[], then start[p] / cnt[p] of every profile (p = P: the invalid bucket).
+__global__ __launch_bounds__(1024) void k_part_base(const int* __restrict__ tot, int nq, int P, int C,
+                                                    int n, int* __restrict__ base,
                                                     int* __restrict__ start, int* __restrict__ cnt) {
   __shared__ int s[1024];
-  const int t = threadIdx.x;
-  const int64_t seg = ((int64_t)m + 1023) / 1024;
-  const int64_t a = min((int64_t)m, t * seg), e = min((int64_t)m, a + seg);
+  const int t = threadIdx.x, seg = (nq + 1023) / 1024;
+  const int a = min(nq, t * seg), e = min(nq, a + seg);
   int sum = 0;
-  for (int64_t i = a; i < e; ++i) sum += bh[i];
+  for (int i = a; i < e; ++i) sum += tot[i];
   s[t] = sum;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
@@ -181,35 +229,32 @@ __global__ __launch_bounds__(1024) void k_part_scan(int* __restrict__ bh, int m,
     __syncthreads();
   }
   int run = s[t] - sum;
-  for (int64_t i = a; i < e; ++i) {
-    const int v = bh[i];
-    bh[i] = run;
-    run += v;
+  for (int i = a; i < e; ++i) {
+    base[i] = run;
+    run += tot[i];
   }
   __syncthreads();
-  for (int q = t; q <= P; q += 1024) {
-    const int st = bh[(size_t)q * nblk];
-    start[q] = st;
-    cnt[q] = (q < P ? bh[(size_t)(q + 1) * nblk] : n) - st;
+  for (int p = t; p <= P; p += 1024) {
+    const int st = base[p * C];
+    start[p] = st;
+    cnt[p] = (p < P ? base[(p + 1) * C] : n) - st;
   }
 }
 
 // Records of the block's curves at the block's bucket offsets (the order
 // inside one block's share of a bucket is whatever the LDS counter gives; no
 // result depends on it).
-__global__ __launch_bounds__(256) void k_part_fill(const int* __restrict__ prof, int n, int P,
-                                                   const int* __restrict__ bh,
-                                                   const double* __restrict__ params,
-                                                   const int* __restrict__ obs,
+__global__ __launch_bounds__(256) void k_part_fill(PartGeom g, int n, const int* __restrict__ bh,
+                                                   const int* __restrict__ base,
                                                    CurveRec* __restrict__ rec) {
   extern __shared__ int h[];
-  for (int q = threadIdx.x; q <= P; q += 256) h[q] = bh[(size_t)q * gridDim.x + blockIdx.x];
+  const int nq = g.P * g.C + 1;
+  for (int q = threadIdx.x; q < nq; q += 256) h[q] = base[q] + bh[(size_t)blockIdx.x * nq + q];
   __syncthreads();
   const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
   for (int c = c0 + threadIdx.x; c < c1; c += 256) {
-    const int p = prof[c];
-    const int pos = atomicAdd(&h[(p >= 0 && p < P) ? p : P], 1);
-    const double* Pc = params + (size_t)c * 7;
+    const int pos = atomicAdd(&h[curve_bucket(g, c)], 1);
+    const double* Pc = g.params + (size_t)c * 7;
     double sn, cs;
     sincos(Pc[2], &sn, &cs);
     CurveRec r;
@@ -221,7 +266,7 @@ __global__ __launch_bounds__(256) void k_part_fill(const int* __restrict__ prof,
     r.w = Pc[5];
     r.dm = Pc[6];
     r.c = c;
-    r.o = obs[c];
+    r.o = g.obs[c];
     rec[pos] = r;
   }
 }
@@ -306,8 +351,13 @@ __global__ __launch_bounds__(256) void k_lc_band(
   const double bsc = bn.bsc, xmax = (double)(Nx - 1), ymax = (double)(Ny - 1);
   const int nn = bn.cnt;
   const double rlo = srad[0], rhi = srad[nn - 1];
-  const int nwaves = gridDim.x * 4;
-  for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w * GPW < ncur; w += nwaves) {  // uniform per wave
+  // every wavefront takes a contiguous run of record groups; logical blocks
+  // are numbered XCD by XCD, so one XCD works through one stretch of the
+  // (map-ordered) records at a time
+  const int nit = (ncur + GPW - 1) / GPW, nwaves = gridDim.x * 4;
+  const int lw = ((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)) * 4 + (threadIdx.x >> 6);
+  const int w_beg = (int)((int64_t)nit * lw / nwaves), w_end = (int)((int64_t)nit * (lw + 1) / nwaves);
+  for (int w = w_beg; w < w_end; ++w) {  // uniform per wave
     const int i = w * GPW + gi;
     const bool act = i < ncur;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -583,36 +633,48 @@ NodeTab upload_tab(mlc_ctx* ctx, const Args& a, Workspace& ws, int io, int ic, i
 
 void run_fused(mlc_ctx* ctx, const Args& a) {
   validate(a);
-  MLC_REQUIRE(a.P <= kMaxProfiles, "lc_chi2_grad: at most 4096 profiles");
+  MLC_REQUIRE(a.P < kMaxBuckets, "lc_chi2_grad: at most 8191 profiles");
   MLC_REQUIRE((int64_t)a.Nx * a.Ny <= INT32_MAX, "lc_chi2_grad: map too large for 32-bit offsets");
   for (int i = 0; i < a.P * a.B; ++i)
     MLC_REQUIRE(a.node_cnt[i] <= kMaxBandNodes, "lc_chi2_grad: at most 16 radius nodes per (profile, band)");
   if (a.ncurves == 0) return;
   const int P = a.P, B = a.B, n = a.ncurves;
+  PartGeom pg{P, 1, 0, 0.0, 0.0, a.prof, a.params, a.obs, a.nobs, B, a.nepoch, a.obs_off, a.t};
+  while (pg.L < 5 && (int64_t)P * pg.C * 4 + 1 <= kMaxBuckets) {
+    ++pg.L;
+    pg.C *= 4;
+  }
+  pg.sx = (double)(1 << pg.L) / a.Nx;
+  pg.sy = (double)(1 << pg.L) / a.Ny;
   const int nblk = (int)cdiv(n, kPartChunk);
-  const int64_t m = (int64_t)(P + 1) * nblk;
-  MLC_REQUIRE(m <= INT32_MAX, "lc_chi2_grad: too many curves x profiles");
+  const int nq = P * pg.C + 1;
+  const int64_t m = (int64_t)nq * nblk;
+  MLC_REQUIRE(m <= INT32_MAX, "lc_chi2_grad: too many curves x buckets");
   Workspace ws(ctx);
   const int ibh = ws.add(sizeof(int) * m), ist = ws.add(sizeof(int) * 2 * (P + 1)),
+            itot = ws.add(sizeof(int) * 2 * (size_t)nq),
             irec = ws.add(sizeof(CurveRec) * (size_t)n), iacc = ws.add(sizeof(double) * 8 * (size_t)n),
             ibad = ws.add(sizeof(int) * (size_t)n), iisg = ws.add(sizeof(double) * std::max(a.nepoch, 1));
   ws.commit();
   int* bh = ws.get<int>(ibh);
   int *start = ws.get<int>(ist), *cnt = start + (P + 1);
+  int *tot = ws.get<int>(itot), *qbase = tot + nq;
   CurveRec* rec = ws.get<CurveRec>(irec);
   double* acc = ws.get<double>(iacc);
   int* bad = ws.get<int>(ibad);
   double* isg = ws.get<double>(iisg);
-  const size_t hl = sizeof(int) * (P + 1);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(256), hl, a.prof, n, P, bh);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_scan, dim3(1), dim3(1024), 0, bh, (int)m, nblk, P, n, start, cnt);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, a.prof, n, P, bh, a.params,
-             a.obs, rec);
+  const size_t hl = sizeof(int) * nq;
+  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(256), hl, pg, n, bh);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, 256)), dim3(256), 0, bh, nq, nblk, tot);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_base, dim3(1), dim3(1024), 0, tot, nq, P, pg.C, n, qbase, start,
+             cnt);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, pg, n, bh, qbase, rec);
   if (a.nepoch > 0)
     MLC_LAUNCH(ctx, "lc_partition", k_inv_sig, dim3(cdiv(a.nepoch, 256)), dim3(256), 0, a.sig, a.nepoch,
                isg);
   constexpr int kCurvesPerBlock = 4 * (64 / kLanes);
-  const int gb = std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, 8));
+  // one resident wave per SIMD slot (the kernel runs 4 waves per SIMD)
+  const int gb = (int)(cdiv(std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, 4)), 8) * 8);
   const size_t mapsz = (size_t)a.Nx * a.Ny;
   const size_t esz = a.precision == MLC_F64 ? 2 * sizeof(double) : 2 * sizeof(float);
   for (int p = 0; p < P; ++p)

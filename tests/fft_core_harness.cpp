@@ -3,8 +3,10 @@
 // header the gfx950 kernels include).  Built by tests/test_fft_core.py with
 // g++ and compared against numpy's FFT.  Test infrastructure only.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
+#include "../microsn_amd/csrc/mlc_ctfft.hpp"
 #include "../microsn_amd/csrc/mlc_fft_core.hpp"
 
 using namespace mlc;
@@ -106,4 +108,52 @@ int h_irfft(const double* X, int N, double* x) {
   }
   return 0;
 }
+}
+
+// ---- compile-time Stockham (mlc_ctfft.hpp), every thread of a stage run in
+// turn before the barrier and again after it
+template <class T, int N, int NS, bool INV>
+static void ct_host_stages(cpx<T>* buf, const cpx<T>* lo, const cpx<T>* hi) {
+  if constexpr (NS < N) {
+    using S = CtStage<T, N, NS, INV>;
+    constexpr int G = CtGeom<N>::G;
+    std::vector<typename std::remove_reference<typename S::Regs>::type> regs(G);
+    for (int t = 0; t < G; ++t) S::load(buf, lo, hi, t, regs[t]);
+    for (int t = 0; t < G; ++t) S::store(buf, t, regs[t]);
+    ct_host_stages<T, N, NS * S::R, INV>(buf, lo, hi);
+  }
+}
+
+template <int N>
+static void ct_host(double* data, int inverse) {
+  using Gm = CtGeom<N>;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  const long double sg = inverse ? 1.0L : -1.0L;
+  std::vector<cpx<double>> lo(Gm::NLO), hi(Gm::NHI), buf(Gm::LDS);
+  for (int l = 0; l < Gm::NLO; ++l)
+    lo[l] = mk((double)std::cos(sg * 2 * pi * l / N), (double)std::sin(sg * 2 * pi * l / N));
+  for (int h = 0; h < Gm::NHI; ++h) {
+    long double a = sg * 2 * pi * (long double)h * Gm::NLO / N;
+    hi[h] = mk((double)std::cos(a), (double)std::sin(a));
+  }
+  for (int i = 0; i < N; ++i) buf[pad16(i)] = mk(data[2 * i], data[2 * i + 1]);
+  if (inverse) ct_host_stages<double, N, 1, true>(buf.data(), lo.data(), hi.data());
+  else ct_host_stages<double, N, 1, false>(buf.data(), lo.data(), hi.data());
+  for (int i = 0; i < N; ++i) {
+    data[2 * i] = buf[pad16(i)].x;
+    data[2 * i + 1] = buf[pad16(i)].y;
+  }
+}
+
+extern "C" int h_ctfft(double* data, int N, int inverse) {
+  switch (N) {
+    case 256: ct_host<256>(data, inverse); return 0;
+    case 512: ct_host<512>(data, inverse); return 0;
+    case 1024: ct_host<1024>(data, inverse); return 0;
+    case 2048: ct_host<2048>(data, inverse); return 0;
+    case 4096: ct_host<4096>(data, inverse); return 0;
+    case 8192: ct_host<8192>(data, inverse); return 0;
+    case 16384: ct_host<16384>(data, inverse); return 0;
+  }
+  return -1;
 }

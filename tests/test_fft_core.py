@@ -25,6 +25,7 @@ def fftlib(tmp_path_factory):
     L.h_fft.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     L.h_rfft.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.h_irfft.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.h_ctfft.argtypes = [C.c_void_p, C.c_int, C.c_int]
     return L
 
 
@@ -53,6 +54,20 @@ def test_complex_fft_matches_numpy(fftlib, L, inverse, quarter):
     got = buf[:, 0] + 1j * buf[:, 1]
     want = L * np.fft.ifft(x) if inverse else np.fft.fft(x)
     assert np.max(np.abs(got - want)) < 1e-13 * np.sqrt(L) * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("N", [256, 512, 1024, 2048, 4096, 8192, 16384])
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_compile_time_fft_matches_numpy(fftlib, N, inverse):
+    """mlc_ctfft.hpp (radix-16 Stockham, two-level twiddle tables, padded
+    LDS indices), run thread by thread around each barrier."""
+    rng = np.random.default_rng(N + inverse)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    buf = np.ascontiguousarray(np.stack([x.real, x.imag], 1))
+    assert fftlib.h_ctfft(_p(buf), N, inverse) == 0
+    got = buf[:, 0] + 1j * buf[:, 1]
+    want = N * np.fft.ifft(x) if inverse else np.fft.fft(x)
+    assert np.max(np.abs(got - want)) < 1e-13 * np.sqrt(N) * np.max(np.abs(want))
 
 
 def test_unsupported_length_is_rejected(fftlib):

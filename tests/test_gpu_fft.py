@@ -88,7 +88,10 @@ def test_convolve_4096_properties(ctx, prec):
     m = torch.full((N, N), 2.0, dtype=torch.float64, device="cuda")
     m[1000, 2000] += 1000.0
     spec = api.MapSpectrum(ctx, m, prec)
-    v = spec.convolve("gaussian", 5.0).double()
+    v, nd = spec.convolve("gaussian", 5.0, node=True)
+    v = v.double()
+    # node maps (compile-time row FFT of the packed value + i derivative) agree with the value map
+    assert float((nd[..., 0].double() - v).abs().max()) < (1e-12 if prec == "f64" else 2e-6) * 1002.0
     mean = float(v.mean())
     assert abs(mean - float(m.mean())) < (1e-12 if prec == "f64" else 1e-6) * float(m.mean())
     # centroid of the excess around the impulse: offset (-1/2, -1/2) px from
@@ -124,7 +127,9 @@ def test_unsupported_sizes_rejected(ctx):
 
 @pytest.mark.parametrize("Ny,Nx,shape,rhalf", [(36, 40, "gaussian", 8.0),     # rows wrap, partial row block
                                                (256, 256, "uniform", 50.0),   # FP64 falls back to column FFTs
-                                               (100, 120, "exponential", 3.1)])
+                                               (100, 120, "exponential", 3.1),
+                                               (64, 1024, "gaussian", 6.0),   # compile-time row FFTs
+                                               (40, 2048, "uniform", 3.0)])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec):
     rng = np.random.default_rng(Ny + Nx)

@@ -63,7 +63,8 @@ def _special_keys(rng, n):
     return k
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 4095, 4096, 4097, 100_000, 1_000_003])
+@pytest.mark.parametrize("n", [1, 2, 5, 1023, 1024, 1025, 4095, 4096, 4097, 100_000, 1_000_003,
+                               2_097_152, 2_097_153])
 def test_sort_matches_stable_sort(ctx, orc, n):
     rng = np.random.default_rng(n)
     keys = _special_keys(rng, n) if n < 100_000 else rng.standard_normal(n) * 10.0 ** rng.integers(-3, 3, n)
@@ -76,6 +77,23 @@ def test_sort_matches_stable_sort(ctx, orc, n):
     assert np.array_equal(p, ref)
     r = rank.cpu().numpy().view(np.uint32)
     assert np.array_equal(r[ref], np.arange(n, dtype=np.uint32))
+    assert np.array_equal(sk.cpu().numpy().view(np.uint64), keys[ref].view(np.uint64))
+
+
+@pytest.mark.parametrize("kind", ["all_equal", "small_ints", "negative_zero"])
+def test_sort_trivial_passes(ctx, orc, kind):
+    """Keys whose digits are constant in some or all passes (those passes copy)."""
+    n = 70_001
+    rng = np.random.default_rng(3)
+    if kind == "all_equal":
+        keys = np.full(n, 2.5)
+    elif kind == "small_ints":
+        keys = rng.integers(0, 7, n).astype(np.float64)
+    else:
+        keys = np.where(rng.random(n) < 0.5, 0.0, -0.0)
+    perm, rank, sk = api.sort_f64(ctx, _cuda(keys), rank=True, sorted_keys=True)
+    ref = orc.stable_argsort(keys)
+    assert np.array_equal(perm.cpu().numpy().view(np.uint32), ref)
     assert np.array_equal(sk.cpu().numpy().view(np.uint64), keys[ref].view(np.uint64))
 
 
