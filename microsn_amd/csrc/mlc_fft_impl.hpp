@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kColThreads) void k_node_col(
 // kYfOut + kYfK - 1 inputs in registers over each block of kYfK taps.  The
 // Nyquist column (u = W-1) is zero.  Logical blocks run down each column
 // strip, XCD by XCD, so the halo rows are shared in one L2.
-constexpr int kYfCols = 64, kYfGroups = 8, kYfOut = 8, kYfK = 4;
+constexpr int kYfCols = 64, kYfGroups = 16, kYfOut = 4, kYfK = 4;
 constexpr int kYfRows = kYfGroups * kYfOut;
 
 template <class T>

@@ -195,19 +195,33 @@ __global__ __launch_bounds__(256) void k_part_count(PartGeom g, int n, int* __re
   for (int q = threadIdx.x; q < nq; q += 256) bh[(size_t)blockIdx.x * nq + q] = h[q];
 }
 
-// Per bucket q (one thread each): exclusive prefix over the blocks, in
-// place, and the bucket total.
-__global__ __launch_bounds__(256) void k_part_colscan(int* __restrict__ bh, int nq, int nblk,
-                                                      int* __restrict__ tot) {
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= nq) return;
+// Exclusive prefix of every bucket's counts over the partition blocks, in
+// place, and the bucket totals.  A workgroup covers 64 buckets; its 16 lane
+// groups take 16 contiguous runs of blocks, whose sums are scanned in LDS.
+constexpr int kColScanQ = 64, kColScanSeg = 16;
+__global__ __launch_bounds__(kColScanQ * kColScanSeg) void k_part_colscan(int* __restrict__ bh, int nq,
+                                                                           int nblk,
+                                                                           int* __restrict__ tot) {
+  __shared__ int s[kColScanSeg][kColScanQ];
+  const int qq = threadIdx.x % kColScanQ, sg = threadIdx.x / kColScanQ;
+  const int q = blockIdx.x * kColScanQ + qq;
+  const int len = (nblk + kColScanSeg - 1) / kColScanSeg;
+  const int b0 = min(nblk, sg * len), b1 = min(nblk, b0 + len);
+  int sum = 0;
+  if (q < nq)
+    for (int b = b0; b < b1; ++b) sum += bh[(size_t)b * nq + q];
+  s[sg][qq] = sum;
+  __syncthreads();
   int run = 0;
-  for (int b = 0; b < nblk; ++b) {
-    const int v = bh[(size_t)b * nq + q];
-    bh[(size_t)b * nq + q] = run;
-    run += v;
+  for (int k = 0; k < sg; ++k) run += s[k][qq];
+  if (q < nq) {
+    for (int b = b0; b < b1; ++b) {
+      const int v = bh[(size_t)b * nq + q];
+      bh[(size_t)b * nq + q] = run;
+      run += v;
+    }
+    if (sg == kColScanSeg - 1) tot[q] = run;
   }
-  tot[q] = run;
 }
 
 // Exclusive scan of the nq bucket totals (one workgroup, nq <= 8192) into
@@ -246,7 +260,8 @@ __global__ __launch_bounds__(1024) void k_part_base(const int* __restrict__ tot,
 // result depends on it).
 __global__ __launch_bounds__(256) void k_part_fill(PartGeom g, int n, const int* __restrict__ bh,
                                                    const int* __restrict__ base,
-                                                   CurveRec* __restrict__ rec) {
+                                                   CurveRec* __restrict__ rec,
+                                                   int* __restrict__ pos_of) {
   extern __shared__ int h[];
   const int nq = g.P * g.C + 1;
   for (int q = threadIdx.x; q < nq; q += 256) h[q] = base[q] + bh[(size_t)blockIdx.x * nq + q];
@@ -268,6 +283,7 @@ __global__ __launch_bounds__(256) void k_part_fill(PartGeom g, int n, const int*
     r.c = c;
     r.o = g.obs[c];
     rec[pos] = r;
+    pos_of[c] = pos;
   }
 }
 
@@ -462,19 +478,20 @@ __global__ __launch_bounds__(256) void k_lc_band(
   }
 }
 
-// chi2 and the 7-parameter gradient of every curve from its accumulator:
+// chi2 and the 7-parameter gradient of every curve (thread = curve, so the
+// outputs are written in order) from the accumulator of its record:
 // d/dtheta = v (cos(theta) [4] - sin(theta) [3]), d/dv = cos(theta) [3] +
 // sin(theta) [4]; records of bucket P (invalid profile) and flagged ones
 // give (+inf, 0).
 __global__ __launch_bounds__(256) void k_lc_finish(int n, int P, const int* __restrict__ start,
-                                                   const CurveRec* __restrict__ rec,
+                                                   const int* __restrict__ pos_of,
                                                    const double* __restrict__ params,
                                                    const double* __restrict__ acc,
                                                    const int* __restrict__ bad,
                                                    double* __restrict__ chi2, double* __restrict__ grad) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int c = rec[i].c;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int i = pos_of[c];
   const bool ok = i < start[P] && !bad[i];
   double* g = grad + (size_t)c * 7;
   if (!ok) {
@@ -654,7 +671,8 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
   const int ibh = ws.add(sizeof(int) * m), ist = ws.add(sizeof(int) * 2 * (P + 1)),
             itot = ws.add(sizeof(int) * 2 * (size_t)nq),
             irec = ws.add(sizeof(CurveRec) * (size_t)n), iacc = ws.add(sizeof(double) * 8 * (size_t)n),
-            ibad = ws.add(sizeof(int) * (size_t)n), iisg = ws.add(sizeof(double) * std::max(a.nepoch, 1));
+            ibad = ws.add(sizeof(int) * (size_t)n), iisg = ws.add(sizeof(double) * std::max(a.nepoch, 1)),
+            ipos = ws.add(sizeof(int) * (size_t)n);
   ws.commit();
   int* bh = ws.get<int>(ibh);
   int *start = ws.get<int>(ist), *cnt = start + (P + 1);
@@ -663,12 +681,14 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
   double* acc = ws.get<double>(iacc);
   int* bad = ws.get<int>(ibad);
   double* isg = ws.get<double>(iisg);
+  int* pos_of = ws.get<int>(ipos);
   const size_t hl = sizeof(int) * nq;
   MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(256), hl, pg, n, bh);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, 256)), dim3(256), 0, bh, nq, nblk, tot);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, kColScanQ)), dim3(kColScanQ * kColScanSeg),
+             0, bh, nq, nblk, tot);
   MLC_LAUNCH(ctx, "lc_partition", k_part_base, dim3(1), dim3(1024), 0, tot, nq, P, pg.C, n, qbase, start,
              cnt);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, pg, n, bh, qbase, rec);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, pg, n, bh, qbase, rec, pos_of);
   if (a.nepoch > 0)
     MLC_LAUNCH(ctx, "lc_partition", k_inv_sig, dim3(cdiv(a.nepoch, 256)), dim3(256), 0, a.sig, a.nepoch,
                isg);
@@ -698,8 +718,8 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
       }
 #undef MLC_BAND
     }
-  MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, P, start, rec, a.params,
-             acc, bad, a.chi2, a.grad);
+  MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, P, start, pos_of,
+             a.params, acc, bad, a.chi2, a.grad);
 }
 
 void run_plain(mlc_ctx* ctx, const Args& a) {
