@@ -367,13 +367,16 @@ __global__ __launch_bounds__(256) void k_lc_band(
   const double bsc = bn.bsc, xmax = (double)(Nx - 1), ymax = (double)(Ny - 1);
   const int nn = bn.cnt;
   const double rlo = srad[0], rhi = srad[nn - 1];
-  // every wavefront takes a contiguous run of record groups; logical blocks
-  // are numbered XCD by XCD, so one XCD works through one stretch of the
-  // (map-ordered) records at a time
-  const int nit = (ncur + GPW - 1) / GPW, nwaves = gridDim.x * 4;
-  const int lw = ((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)) * 4 + (threadIdx.x >> 6);
-  const int w_beg = (int)((int64_t)nit * lw / nwaves), w_end = (int)((int64_t)nit * (lw + 1) / nwaves);
-  for (int w = w_beg; w < w_end; ++w) {  // uniform per wave
+  // Workgroups are dispatched round-robin over the 8 XCDs, so blockIdx & 7
+  // names the XCD.  Each XCD owns one contiguous eighth of the (map-ordered)
+  // record groups and its wavefronts stride through that range together:
+  // the groups in flight on one XCD at any moment are neighbours on the map
+  // and share its L2.
+  const int nit = (ncur + GPW - 1) / GPW;
+  const int xcd = blockIdx.x & 7, nwx = (gridDim.x >> 3) * 4;
+  const int wx = (blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);
+  const int g0 = (int)((int64_t)nit * xcd / 8), g1 = (int)((int64_t)nit * (xcd + 1) / 8);
+  for (int w = g0 + wx; w < g1; w += nwx) {  // uniform per wave
     const int i = w * GPW + gi;
     const bool act = i < ncur;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
