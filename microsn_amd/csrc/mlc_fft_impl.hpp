@@ -345,6 +345,7 @@ __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
     int Nx, int Ny, int W) {
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem);  // [kYfRows + 2 hp][kYfCols]
+  cpx<T>* taps = tile + (size_t)(kYfRows + 2 * hp) * kYfCols;  // [hp][kYfCols] (value, deriv)
   const int nstrip = (W + kYfCols - 1) / kYfCols, nyb = (Ny + kYfRows - 1) / kYfRows;
   const int lb = xcd_block(nstrip * nyb);
   if (lb >= nstrip * nyb) return;
@@ -367,6 +368,11 @@ __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
     for (int e = 0; e < 4; ++e)
       if (r0 + e * kYfGroups < nrow) tile[(r0 + e * kYfGroups) * kYfCols + c] = v[e];
   }
+  // the block's taps, once, in the working precision (zero beyond h)
+  for (int k = g; k < hp; k += kYfGroups)
+    taps[k * kYfCols + c] = (col_in && k < h)
+                                ? mk<T>((T)av[(size_t)k * W + u], out_d ? (T)ad[(size_t)k * W + u] : (T)0)
+                                : mk<T>(0, 0);
   __syncthreads();
   if (u >= W) return;
   const int ybase = y0 + g * kYfOut;
@@ -387,9 +393,9 @@ __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
     T bv[kYfK], bd[kYfK];
 #pragma unroll
     for (int kk = 0; kk < kYfK; ++kk) {
-      const int k = kb + kk;
-      bv[kk] = k < h ? (T)av[(size_t)k * W + u] : (T)0;
-      bd[kk] = (out_d && k < h) ? (T)ad[(size_t)k * W + u] : (T)0;
+      const cpx<T> tk = taps[(kb + kk) * kYfCols + c];
+      bv[kk] = tk.x;
+      bd[kk] = tk.y;
     }
 #pragma unroll
     for (int kk = 0; kk < kYfK; ++kk)
@@ -521,9 +527,13 @@ __global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ p
   if (u >= W) return;
   const int s1 = h + k, s2 = h - 1 - k, twoN = 2 * Nx;
   double sa = 0.0, sd = 0.0;
+  // angle index u (2(cc-h)+1) mod 2Nx, advanced by 2u per column
+  int m = (int)(((long long)u * (1 - 2 * h) % twoN + twoN) % twoN);
+  const int step = (int)((2LL * u) % twoN);
   for (int cc = 0; cc < n; ++cc) {
-    const int m = (int)(((long long)u * (2 * (cc - h) + 1) % twoN + twoN) % twoN);
     const double w = cosx[m];
+    m += step;
+    if (m >= twoN) m -= twoN;
     sa += (prof[s1 * n + cc] + prof[s2 * n + cc]) * w;
     sd += (dprof[s1 * n + cc] + dprof[s2 * n + cc]) * w;
   }
@@ -854,7 +864,7 @@ template <class T>
 bool node_yfilter(mlc_ctx* ctx, const cpx<T>* rows, const double* av, const double* ad, int h,
                   cpx<T>* ov, cpx<T>* od, int Nx, int Ny, int W) {
   const int hp = (h + kYfK - 1) / kYfK * kYfK;
-  const size_t lds = sizeof(cpx<T>) * (size_t)(kYfRows + 2 * hp) * kYfCols;
+  const size_t lds = sizeof(cpx<T>) * (size_t)(kYfRows + 3 * hp) * kYfCols;
   if (lds > kLdsBytes) return false;
   const int nb = ((W + kYfCols - 1) / kYfCols) * ((Ny + kYfRows - 1) / kYfRows);
   auto k = k_node_yfilter<T>;
