@@ -7,6 +7,8 @@
 #ifndef MLC_HOST_H
 #define MLC_HOST_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -43,6 +45,35 @@ int mlc_mpd_divide_by(int NA, const double* binsA, const double* countsA, int NB
                       double* countsR);
 /* trapezium (integration.cpp:3-9; integration.hpp). */
 double mlc_trapezium(int N, const double* f1, const double* f2, const double* x);
+
+/* GERLUMPH map directory (magnification_map.cpp:7-40): <dir>/mapmeta.dat
+ * ("avgmu avgN / Nx / width / k g s") and <dir>/map.bin (Nx*Nx native int32
+ * ray counts).  Fills the metadata; with counts != NULL (capacity cap >=
+ * Nx*Nx) also reads the counts.  Returns 1 for bad arguments, 2 for missing
+ * or malformed metadata, 3 for a missing or short map.bin (the reference does
+ * not check either file). */
+int mlc_gerlumph_read(const char* dir, double* avgmu, double* avgN, int* Nx, double* width,
+                      double* k, double* g, double* s, int32_t* counts, int64_t cap);
+/* Writes a map directory in the same format (no reference counterpart; for
+ * synthetic maps). */
+int mlc_gerlumph_write(const char* dir, double avgmu, double avgN, int Nx, double width, double k,
+                       double g, double s, const int32_t* counts);
+/* Image::writeImageFITS (image.cpp:20-62): every sampling-th pixel of an
+ * Ny x Nx FP64 image as FLOAT_IMG, rows flipped, in the primary HDU and in an
+ * image extension "NEW-EXTENSION". */
+int mlc_write_fits(const char* path, const double* data, int Nx, int Ny, int sampling);
+/* Image::writeImageBIN (image.cpp:4-18): the same samples as native FP64. */
+int mlc_write_bin(const char* path, const double* data, int Nx, int Ny, int sampling);
+
+/* BaseProfile size laws (profile.cpp:86-124) and their throughput-weighted
+ * forms over a filter curve (profile.cpp:55-84, trapezoid rule). */
+double mlc_size_parametric(double r0, double l0, double nu, double lrest);
+double mlc_size_ss(double mbh, double fedd, double eta, double lrest);
+double mlc_size_parametric_band(double r0, double l0, double nu, const double* wavelength,
+                                const double* throughput, int n);
+double mlc_size_ss_band(double mbh, double fedd, double eta, const double* wavelength,
+                        const double* throughput, int n);
+double mlc_size_vector_band(const double* wavelength, const double* throughput, int n);
 
 #ifdef __cplusplus
 }

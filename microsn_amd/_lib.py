@@ -73,6 +73,17 @@ SIGNATURES = {
     "mlc_mpd_normalize": (_i, [_i, _p, _p]),
     "mlc_mpd_divide_by": (_i, [_i, _p, _p, _i, _p, _p, _i, _p, _p]),
     "mlc_trapezium": (_d, [_i, _p, _p, _p]),
+    "mlc_gerlumph_read": (_i, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                               C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double), C.POINTER(C.c_double), _p, _i64]),
+    "mlc_gerlumph_write": (_i, [C.c_char_p, _d, _d, _i, _d, _d, _d, _d, _p]),
+    "mlc_write_fits": (_i, [C.c_char_p, _p, _i, _i, _i]),
+    "mlc_write_bin": (_i, [C.c_char_p, _p, _i, _i, _i]),
+    "mlc_size_parametric": (_d, [_d, _d, _d, _d]),
+    "mlc_size_ss": (_d, [_d, _d, _d, _d]),
+    "mlc_size_parametric_band": (_d, [_d, _d, _d, _p, _p, _i]),
+    "mlc_size_ss_band": (_d, [_d, _d, _d, _p, _p, _i]),
+    "mlc_size_vector_band": (_d, [_p, _p, _i]),
 }
 
 _lib = None

@@ -378,3 +378,37 @@ def mpd_divide_by(bA, cA, bB, cB, bR):
 def trapezium(f1, f2, x):
     f1, f2, x = (np.ascontiguousarray(a, np.float64) for a in (f1, f2, x))
     return lib().mlc_trapezium(x.size, ptr(f1), ptr(f2), ptr(x))
+
+
+# ------------------------------------------------------------------ source sizes
+def _stof(v):
+    """std::stof of the string form of a parameter (profile.cpp:36-48)."""
+    return float(np.float32(float(v)))
+
+
+def get_size(pars, lrest=None, wavelength=None, throughput=None):
+    """BaseProfile::getSize (profile.cpp:31-84): half-light radius from the
+    size law in pars["type"] ("parametric": r0, l0, nu; "ss_disc": mbh, fedd,
+    eta; "vector"; "custom"), at rest wavelength lrest or averaged over a
+    filter curve (wavelength, throughput; trapezoid rule).  Parameters are
+    read through std::stof as in the reference.  An unknown type gives 0."""
+    t = pars.get("type")
+    band = wavelength is not None
+    if band:
+        wl = np.ascontiguousarray(wavelength, np.float64)
+        tp = np.ascontiguousarray(throughput, np.float64)
+        if wl.shape != tp.shape or wl.size < 2:
+            raise ValueError("wavelength and throughput must be equal-length arrays of >= 2 points")
+    if t == "parametric":
+        r0, l0, nu = (_stof(pars[k]) for k in ("r0", "l0", "nu"))
+        return (lib().mlc_size_parametric_band(r0, l0, nu, ptr(wl), ptr(tp), wl.size) if band
+                else lib().mlc_size_parametric(r0, l0, nu, float(lrest)))
+    if t == "ss_disc":
+        mbh, fedd, eta = (_stof(pars[k]) for k in ("mbh", "fedd", "eta"))
+        return (lib().mlc_size_ss_band(mbh, fedd, eta, ptr(wl), ptr(tp), wl.size) if band
+                else lib().mlc_size_ss(mbh, fedd, eta, float(lrest)))
+    if t == "vector":
+        return lib().mlc_size_vector_band(ptr(wl), ptr(tp), wl.size) if band else float(lrest)
+    if t == "custom":
+        return 1.0
+    return 0.0

@@ -4,7 +4,10 @@
 // same C ABI as the kernels.
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "../../include/mlc_host.h"
@@ -158,6 +161,184 @@ double mlc_trapezium(int N, const double* f1, const double* f2, const double* x)
   double acc = 0.0;
   for (int i = 1; i < N; ++i) acc += (f1[i - 1] * f2[i - 1] + f1[i] * f2[i]) * (x[i] - x[i - 1]) / 2.0;
   return acc;
+}
+
+// ------------------------------------------------------------------ map I/O
+// GERLUMPH map directory (magnification_map.cpp:7-40): mapmeta.dat holds
+// "avgmu avgN / Nx / width / k g s" (whitespace separated), map.bin Nx*Nx
+// native-endian int32 ray counts.  Unlike the reference, unreadable or short
+// files are reported.
+int mlc_gerlumph_read(const char* dir, double* avgmu, double* avgN, int* Nx, double* width,
+                      double* k, double* g, double* s, int32_t* counts, int64_t cap) {
+  if (!dir || !avgmu || !avgN || !Nx || !width || !k || !g || !s) return 1;
+  const std::string base(dir);
+  FILE* f = std::fopen((base + "/mapmeta.dat").c_str(), "r");
+  if (!f) return 2;
+  const int got = std::fscanf(f, "%lf %lf %d %lf %lf %lf %lf", avgmu, avgN, Nx, width, k, g, s);
+  std::fclose(f);
+  if (got != 7 || *Nx <= 0) return 2;
+  if (!counts) return 0;
+  const int64_t n = (int64_t)*Nx * *Nx;
+  if (cap < n) return 1;
+  FILE* b = std::fopen((base + "/map.bin").c_str(), "rb");
+  if (!b) return 3;
+  const size_t rd = std::fread(counts, sizeof(int32_t), (size_t)n, b);
+  std::fclose(b);
+  return rd == (size_t)n ? 0 : 3;
+}
+
+// Writes a map directory in the same format (synthetic maps, tests).
+int mlc_gerlumph_write(const char* dir, double avgmu, double avgN, int Nx, double width, double k,
+                       double g, double s, const int32_t* counts) {
+  if (!dir || Nx <= 0 || !counts) return 1;
+  const std::string base(dir);
+  FILE* f = std::fopen((base + "/mapmeta.dat").c_str(), "w");
+  if (!f) return 2;
+  std::fprintf(f, "%.17g %.17g\n%d\n%.17g\n%.17g %.17g %.17g\n", avgmu, avgN, Nx, width, k, g, s);
+  std::fclose(f);
+  FILE* b = std::fopen((base + "/map.bin").c_str(), "wb");
+  if (!b) return 3;
+  const size_t n = (size_t)Nx * Nx;
+  const size_t wr = std::fwrite(counts, sizeof(int32_t), n, b);
+  std::fclose(b);
+  return wr == n ? 0 : 3;
+}
+
+namespace {
+// One 80-character FITS header card.
+void fits_card(std::string& h, const std::string& key, const std::string& value) {
+  char c[81];  // fixed format: strings start in column 11, numbers end in column 30
+  if (!value.empty() && value[0] == '\'')
+    std::snprintf(c, sizeof(c), "%-8.8s= %-20s", key.c_str(), value.c_str());
+  else
+    std::snprintf(c, sizeof(c), "%-8.8s= %20s", key.c_str(), value.c_str());
+  std::string card(c);
+  card.resize(80, ' ');
+  h += card;
+}
+void fits_end(std::string& h) {
+  std::string card("END");
+  card.resize(80, ' ');
+  h += card;
+  h.resize((h.size() + 2879) / 2880 * 2880, ' ');
+}
+// Big-endian float32 data unit, zero-padded to 2880 bytes.
+void fits_data(std::string& out, const std::vector<float>& v) {
+  const size_t start = out.size();
+  out.resize(start + v.size() * 4);
+  for (size_t i = 0; i < v.size(); ++i) {
+    uint32_t u;
+    std::memcpy(&u, &v[i], 4);
+    out[start + 4 * i + 0] = (char)(u >> 24);
+    out[start + 4 * i + 1] = (char)(u >> 16);
+    out[start + 4 * i + 2] = (char)(u >> 8);
+    out[start + 4 * i + 3] = (char)u;
+  }
+  out.resize((out.size() + 2879) / 2880 * 2880, '\0');
+}
+// Samples every `sampling`-th row and column (the first nNy = Ny/sampling rows
+// and nNx = Nx/sampling columns).
+std::vector<double> subsample(const double* data, int Nx, int Ny, int sampling, int* nNx, int* nNy) {
+  *nNx = Nx / sampling;
+  *nNy = Ny / sampling;
+  std::vector<double> o((size_t)*nNx * *nNy);
+  for (int i = 0; i < *nNy; ++i)
+    for (int j = 0; j < *nNx; ++j) o[(size_t)i * *nNx + j] = data[(size_t)i * sampling * Nx + (size_t)j * sampling];
+  return o;
+}
+}  // namespace
+
+// Image::writeImageFITS (image.cpp:20-62): the sub-sampled image as FLOAT_IMG,
+// rows flipped (first stored row = last image row), in the primary HDU and
+// again in an image extension named NEW-EXTENSION.  The reference gives the
+// extension the axes (nNy, nNy); that equals (nNx, nNy) for square images,
+// which is what is written for all.
+int mlc_write_fits(const char* path, const double* data, int Nx, int Ny, int sampling) {
+  if (!path || !data || Nx <= 0 || Ny <= 0 || sampling <= 0 || Nx / sampling < 1 || Ny / sampling < 1)
+    return 1;
+  int nNx, nNy;
+  const std::vector<double> o = subsample(data, Nx, Ny, sampling, &nNx, &nNy);
+  std::vector<float> arr(o.size());
+  for (int i = 0; i < nNy; ++i)
+    for (int j = 0; j < nNx; ++j) arr[(size_t)(nNy - 1 - i) * nNx + j] = (float)o[(size_t)i * nNx + j];
+  std::string out;
+  fits_card(out, "SIMPLE", "T");
+  fits_card(out, "BITPIX", "-32");
+  fits_card(out, "NAXIS", "2");
+  fits_card(out, "NAXIS1", std::to_string(nNx));
+  fits_card(out, "NAXIS2", std::to_string(nNy));
+  fits_card(out, "EXTEND", "T");
+  fits_end(out);
+  fits_data(out, arr);
+  fits_card(out, "XTENSION", "'IMAGE   '");
+  fits_card(out, "BITPIX", "-32");
+  fits_card(out, "NAXIS", "2");
+  fits_card(out, "NAXIS1", std::to_string(nNx));
+  fits_card(out, "NAXIS2", std::to_string(nNy));
+  fits_card(out, "PCOUNT", "0");
+  fits_card(out, "GCOUNT", "1");
+  fits_card(out, "EXTNAME", "'NEW-EXTENSION'");
+  fits_end(out);
+  fits_data(out, arr);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 2;
+  const size_t wr = std::fwrite(out.data(), 1, out.size(), f);
+  std::fclose(f);
+  return wr == out.size() ? 0 : 2;
+}
+
+// Image::writeImageBIN (image.cpp:4-18): the sub-sampled image as native FP64.
+int mlc_write_bin(const char* path, const double* data, int Nx, int Ny, int sampling) {
+  if (!path || !data || Nx <= 0 || Ny <= 0 || sampling <= 0 || Nx / sampling < 1 || Ny / sampling < 1)
+    return 1;
+  int nNx, nNy;
+  const std::vector<double> o = subsample(data, Nx, Ny, sampling, &nNx, &nNy);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 2;
+  const size_t wr = std::fwrite(o.data(), sizeof(double), o.size(), f);
+  std::fclose(f);
+  return wr == o.size() ? 0 : 2;
+}
+
+// ------------------------------------------------------------------ source sizes
+// BaseProfile size laws (profile.cpp:86-124), half-light radius in the units
+// of r0 (parametric) or 10^14 cm (Shakura-Sunyaev disc).
+double mlc_size_parametric(double r0, double l0, double nu, double lrest) {
+  return r0 * std::pow(lrest / l0, nu);
+}
+double mlc_size_ss(double mbh, double fedd, double eta, double lrest) {
+  return 0.0097 * std::pow(std::pow(lrest, 4.0) * std::pow(mbh, 2.0) * (fedd / eta), 1.0 / 3.0);
+}
+// Throughput-weighted sizes over a filter curve (trapezoid rule), and the
+// throughput-weighted mean wavelength of the "vector" type (profile.cpp:55-84).
+double mlc_size_parametric_band(double r0, double l0, double nu, const double* wl, const double* tp,
+                                int n) {
+  double reff = 0.0, norm = 0.0;
+  for (int i = 1; i < n; ++i) {
+    norm += (wl[i] - wl[i - 1]) * (tp[i] + tp[i - 1]) / 2.0;
+    const double a = r0 * std::pow(wl[i - 1] / l0, nu), b = r0 * std::pow(wl[i] / l0, nu);
+    reff += (wl[i] - wl[i - 1]) * (tp[i - 1] * a + tp[i] * b) / 2.0;
+  }
+  return reff / norm;
+}
+double mlc_size_ss_band(double mbh, double fedd, double eta, const double* wl, const double* tp, int n) {
+  const double b2 = std::pow(mbh, 2.0), c = fedd / eta;
+  double reff = 0.0, norm = 0.0;
+  for (int i = 1; i < n; ++i) {
+    norm += (wl[i] - wl[i - 1]) * (tp[i] + tp[i - 1]) / 2.0;
+    const double a = 0.0097 * std::pow(std::pow(wl[i - 1], 4.0) * b2 * c, 1.0 / 3.0);
+    const double b = 0.0097 * std::pow(std::pow(wl[i], 4.0) * b2 * c, 1.0 / 3.0);
+    reff += (wl[i] - wl[i - 1]) * (tp[i - 1] * a + tp[i] * b) / 2.0;
+  }
+  return reff / norm;
+}
+double mlc_size_vector_band(const double* wl, const double* tp, int n) {
+  double leff = 0.0, norm = 0.0;
+  for (int i = 1; i < n; ++i) {
+    norm += (wl[i] - wl[i - 1]) * (tp[i] + tp[i - 1]) / 2.0;
+    leff += (wl[i] - wl[i - 1]) * (tp[i] * wl[i] + tp[i - 1] * wl[i - 1]) / 2.0;
+  }
+  return leff / norm;
 }
 
 }  // extern "C"
