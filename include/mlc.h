@@ -61,6 +61,9 @@ int mlc_free(mlc_ctx* ctx, void* d_ptr);
 int mlc_memcpy_h2d(mlc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int mlc_memcpy_d2h(mlc_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int mlc_memset(mlc_ctx* ctx, void* d_dst, int value, size_t bytes);
+/* Free and total bytes of the context's device (the reference's
+ * get_free_gpu_mem / get_total_gpu_mem, util.cu:125-163). */
+int mlc_mem_info(mlc_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* Per-kernel HIP-event timing on the context's stream.  mlc_timing_get
  * synchronises and returns the summed duration and launch count of every
  * launch of the named kernel since the last reset. */
@@ -77,12 +80,14 @@ int mlc_gerlumph_normalise(mlc_ctx* ctx, const int32_t* d_counts, int64_t npix,
                            double avgmu, double avgN, double k, double g,
                            double* d_map);
 
-/* Spectrum of a map: the forward 2-D real FFT of an Ny x Nx row-major FP64
- * map (cuFFT D2Z, gpu_functions.cu:151,175,522-526), scaled by 1/(Nx*Ny) (the
- * normalisation of kernel_multiplyFFTs, gpu_functions.cu:571-577) and held in
- * `precision`.  With subtract_mean != 0 the FP64 mean is removed first and
- * added back to every convolved value map (the kernel sums to one). Nx and Ny
- * must be even, Nx/2 and Ny with prime factors in {2,3,5} only. */
+/* A map held for convolution (the map side of cuFFT D2Z,
+ * gpu_functions.cu:151,175,522-526): the real FFT of every row of an Ny x Nx
+ * row-major FP64 map, in `precision`; the full 2-D spectrum scaled by
+ * 1/(Nx*Ny) (the normalisation of kernel_multiplyFFTs, gpu_functions.cu:
+ * 571-577) is built from it when a call needs it.  With subtract_mean != 0
+ * the FP64 mean is removed first and added back to every convolved value map
+ * (the kernel sums to one).  Nx and Ny must be even, Nx/2 and Ny with prime
+ * factors in {2,3,5} only. */
 int mlc_map_create(mlc_ctx* ctx, int precision, const double* d_map, int Nx,
                    int Ny, int subtract_mean, mlc_map** out);
 void mlc_map_destroy(mlc_map* map);
@@ -106,9 +111,12 @@ int mlc_profile_size(int shape, double rhalf, double pix, double rin, double a);
 
 /* Convolution of the map with one profile: Kernel::setKernel
  * (magnification_map.cpp:172-194) + D2Z of the kernel + kernel_multiplyFFTs +
- * Z2D (gpu_functions.cu:184-200), fused: the kernel spectrum is built on the
- * device from the n x n profile, the pointwise product is taken inside the
- * column FFT and nothing of size Nx*Ny is staged from the host.
+ * Z2D (gpu_functions.cu:184-200), fused: the n x n profile is built on the
+ * device, its symmetry about the pixel corner turns the column pass of the
+ * 2-D convolution into a 2h-tap filter along y on the map's row spectrum,
+ * and one inverse row FFT per row gives the result; nothing of size Nx*Ny is
+ * staged from the host and no column FFT runs (kernels wider than the
+ * staging limit fall back to fused column FFTs).
  *   d_value: Ny x Nx map in the handle's precision (float or double), or NULL
  *   d_node : Ny x Nx interleaved (value, d value / d rhalf) pairs, or NULL;
  *            stored in 4 x 4 pixel tiles: pixel (x, y) is pair number

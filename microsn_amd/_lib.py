@@ -32,6 +32,7 @@ SIGNATURES = {
     "mlc_memcpy_h2d": (_i, [_p, _p, _p, _sz]),
     "mlc_memcpy_d2h": (_i, [_p, _p, _p, _sz]),
     "mlc_memset": (_i, [_p, _p, _i, _sz]),
+    "mlc_mem_info": (_i, [_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mlc_timing_enable": (_i, [_p, _i]),
     "mlc_timing_reset": (_i, [_p]),
     "mlc_timing_get": (_i, [_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
@@ -104,6 +105,12 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `make -C microsn_amd/csrc` "
                               "(or __graft_entry__.build())")
+        # torch ships its own HIP runtime; importing it first makes libmlc.so
+        # bind to that one instead of loading a second runtime into the process
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -171,6 +171,10 @@ int run(const std::string& input_path) {
 
   // ---- device
   check(mlc_ctx_create(0, nullptr, &g_ctx), "mlc_ctx_create");
+  size_t free_b = 0, total_b = 0;
+  check(mlc_mem_info(g_ctx, &free_b, &total_b), "mlc_mem_info");
+  // driver.cpp:110-111 prints get_total_gpu_mem(), which is in MiB
+  std::printf("Total GPU memory: %ld (bytes)\n", (long)(total_b / (1024 * 1024)));
   Map A = load_map(dirA), B = load_map(dirB);
   if (A.Nx != Nmap || B.Nx != Nmap)
     std::fprintf(stderr, "warning: the maps have %d and %d pixels a side, \"Nmap\" says %d\n", A.Nx, B.Nx,

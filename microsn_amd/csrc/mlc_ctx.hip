@@ -167,6 +167,13 @@ int mlc_memset(mlc_ctx* ctx, void* d_dst, int value, size_t bytes) {
   });
 }
 
+int mlc_mem_info(mlc_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(free_bytes && total_bytes, "mlc_mem_info: null output");
+    hip_check(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+  });
+}
+
 int mlc_timing_enable(mlc_ctx* ctx, int on) {
   return guarded(ctx, [&] {
     if (!on) timing_flush(ctx);

@@ -103,3 +103,23 @@ def test_load_map_on_device_is_bitexact(tmp_path, ctx):
     mapio.write_gerlumph(d, counts, avgmu=2.1, avgN=137.0, width=10.0, k=0.45, g=0.3)
     meta, dmap = mapio.load_map(ctx, d)
     assert np.array_equal(dmap.cpu().numpy(), mapio.magnification(meta, counts))
+
+
+def test_chi2_driver_output():
+    """chi2_driver.cu:8-79 with N = 1000: stdout in the reference's order, the
+    likelihood N^2 exp(-10.0070457) of the constant fake data from both the
+    device run (the kernel's indexing) and the host run (the CPU twin's)."""
+    exe = os.path.join(ROOT, "microsn_amd", "chi2_driver")
+    assert os.path.exists(exe), "chi2_driver not built"
+    r = subprocess.run([exe, "1000", "7", "10"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[0] == "GPU run:" and lines[1].startswith("Elapsed time: ")
+    assert lines[2].startswith("Likelihood: ")
+    assert lines[3].startswith("number of free bytes in gpu memory: ")
+    assert lines[4] == "CPU run:" and lines[5].startswith("Elapsed time: ")
+    assert lines[6].startswith("Likelihood: ")
+    chi2 = float(np.float32(35 * ((0.97 - 0.42 / 1.1) / 1.1) ** 2))
+    want = 1000 * 1000 * np.exp(-chi2)
+    for line in (lines[2], lines[6]):
+        assert abs(float(line.split()[1]) - want) < 1e-5 * want
