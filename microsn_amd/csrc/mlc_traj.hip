@@ -287,10 +287,39 @@ __global__ __launch_bounds__(256) void k_part_fill(PartGeom g, int n, const int*
   }
 }
 
-__global__ __launch_bounds__(256) void k_inv_sig(const double* __restrict__ sig, int nepoch,
-                                                 double* __restrict__ isg) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j < nepoch) isg[j] = 1.0 / sig[j];
+// One epoch as the band kernels read it: time, observed magnitude, 1/sigma.
+struct alignas(32) Epoch {
+  double t, m, is, pad;
+};
+
+// The epoch records, and per (observation set o, band b) segment the
+// earliest and latest epoch time (NaN when the segment holds a NaN time).
+// One wavefront per segment; a segment whose range is invalid is skipped,
+// the band kernels flag its curves before reading anything of it.
+__global__ __launch_bounds__(256) void k_epoch_prep(const int* __restrict__ obs_off, int nseg,
+                                                    int nepoch, const double* __restrict__ tt,
+                                                    const double* __restrict__ mobs,
+                                                    const double* __restrict__ sig,
+                                                    Epoch* __restrict__ ep, double2* __restrict__ rng) {
+  const int sgi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (sgi >= nseg) return;  // uniform per wave
+  const int e0 = obs_off[sgi], e1 = obs_off[sgi + 1];
+  if (!(e0 >= 0 && e1 <= nepoch && e0 <= e1)) return;
+  double lo = INFINITY, hi = -INFINITY;
+  bool nan = false;
+  for (int j = e0 + lane; j < e1; j += 64) {
+    const double t = tt[j];
+    ep[j] = Epoch{t, mobs[j], 1.0 / sig[j], 0.0};
+    nan |= t != t;
+    lo = fmin(lo, t);
+    hi = fmax(hi, t);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  const bool any_nan = __ballot(nan) != 0;
+  if (lane == 0) rng[sgi] = any_nan ? make_double2(NAN, NAN) : make_double2(lo, hi);
 }
 
 // The radius nodes of one (profile, band), passed by value.
@@ -337,28 +366,41 @@ __device__ __forceinline__ double group_sum8(double (&v)[8], int li) {
 
 // One (profile p, band b) launch.  A group of kLanes lanes per curve record
 // (64/kLanes curves per wavefront, grid-stride over the records of bucket
-// p), the band-b epochs strided across the group.  Per epoch:
+// p), the band-b epochs strided across the group.  A curve is checked once
+// against the pixel-centre hull at the earliest and latest epoch of the band:
+// its position is monotone in t, so every epoch in between lies inside as
+// well, and a curve outside is flagged and skips the band.  Per epoch:
 //   mu, dmu/dx, dmu/dy, dmu/dr  from the bilinear taps of the nodes (T)
 //   rho = (mobs - dm + 2.5 log10 mu) / sig,  q = rho / sig        (FP64)
 //   dc = d chi2 / d mu = 5 q / (ln 10 mu)
 // and the group sums, in FP64,
 //   [0] rho^2  [1] dc mx  [2] dc my  [3] dc t mx  [4] dc t my
 //   [5] bsc dc mr  [6] bsc dc t mr  [7] -2 q
-// go to the record's accumulator (band 0 writes, later bands add).
-template <class T, bool FIRST>
+// go to the record's accumulator (band 0 writes, later bands add).  ONE: the
+// (profile, band) has a single radius node (first-order Taylor step in r).
+template <class T, bool FIRST, bool ONE>
 __global__ __launch_bounds__(256) void k_lc_band(
     const cpx<T>* __restrict__ base, int Nx, int Ny, BandNodes bn, int p, int b, int B,
     const CurveRec* __restrict__ rec, const int* __restrict__ start, const int* __restrict__ cnt,
-    int nobs, const int* __restrict__ obs_off, int nepoch, const double* __restrict__ tt,
-    const double* __restrict__ mobs, const double* __restrict__ isg, double* __restrict__ acc,
-    int* __restrict__ bad) {
+    int nobs, const int* __restrict__ obs_off, int nepoch, const Epoch* __restrict__ ep,
+    const double2* __restrict__ rng, double* __restrict__ acc, int* __restrict__ bad) {
   constexpr int GPW = 64 / kLanes;
+  constexpr bool kF32 = sizeof(T) == 4;
   constexpr double kMag = -1.08573620475812959810;  // -2.5 / ln 10
   constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
-  __shared__ double srad[kMaxBandNodes];
-  if (threadIdx.x == 0) {
+  __shared__ double srad[ONE ? 1 : kMaxBandNodes], sih[ONE ? 1 : kMaxBandNodes];
+  __shared__ LnEntry lnt[kF32 ? kLnTab : 1];
+  if constexpr (!ONE) {
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < kMaxBandNodes; ++q) srad[q] = bn.rad[q];
+      for (int q = 0; q < kMaxBandNodes; ++q) {
+        srad[q] = bn.rad[q];
+        sih[q] = q + 1 < bn.cnt ? 1.0 / (bn.rad[q + 1] - bn.rad[q]) : 0.0;
+      }
+    }
+  }
+  if constexpr (kF32) {
+    for (int q = threadIdx.x; q < kLnTab; q += 256) lnt[q] = ln_entry(q);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, gi = lane / kLanes, li = lane % kLanes;
@@ -366,7 +408,9 @@ __global__ __launch_bounds__(256) void k_lc_band(
   const size_t mapsz = (size_t)Nx * Ny;
   const double bsc = bn.bsc, xmax = (double)(Nx - 1), ymax = (double)(Ny - 1);
   const int nn = bn.cnt;
-  const double rlo = srad[0], rhi = srad[nn - 1];
+  const double rlo = bn.rad[0], rhi = ONE ? rlo : srad[nn - 1];
+  // one row of 4 x 4 tiles: the pair-offset step from pixel row 4k+3 to 4k+4
+  const int rowstep = 4 * Nx - 12;
   // Workgroups are dispatched round-robin over the 8 XCDs, so blockIdx & 7
   // names the XCD.  Each XCD owns one contiguous eighth of the (map-ordered)
   // record groups and its wavefronts stride through that range together:
@@ -387,23 +431,33 @@ __global__ __launch_bounds__(256) void k_lc_band(
       const bool okobs = o >= 0 && o < nobs;
       const int e0 = okobs ? obs_off[(size_t)o * B + b] : 0;
       const int e1 = okobs ? obs_off[(size_t)o * B + b + 1] : 0;
-      flag = !(okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1);
-      const int jend = flag ? e0 : e1;
-      for (int j = e0 + li; j < jend; j += kLanes) {
-        const double te = tt[j];
+      bool ok = okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1;
+      if (ok && e0 < e1) {
+        const double2 tr = rng[(size_t)o * B + b];
+        const double xa = fma(R.vct, tr.x, R.x0), xb = fma(R.vct, tr.y, R.x0);
+        const double ya = fma(R.vst, tr.x, R.y0), yb = fma(R.vst, tr.y, R.y0);
+        ok = xa >= 0.0 && xa <= xmax && xb >= 0.0 && xb <= xmax && ya >= 0.0 && ya <= ymax &&
+             yb >= 0.0 && yb <= ymax;
+      }
+      flag = !ok;
+      const int jend = ok ? e1 : e0;
+      int j = e0 + li;
+      Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
+      for (; j < jend; j += kLanes) {
+        const Epoch E = en;
+        if (j + kLanes < jend) en = ep[j + kLanes];
+        const double te = E.t;
+        // inside the hull (checked above): 0 <= x <= Nx - 1, 0 <= y <= Ny - 1
         const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
         const double r = bsc * fma(R.w, te, R.r0);
-        const bool in = x >= 0.0 && x <= xmax && y >= 0.0 && y <= ymax;
-        const double xs = in ? x : 0.0, ys = in ? y : 0.0;
-        const int ix = min((int)xs, Nx - 2), iy = min((int)ys, Ny - 2);
-        const T fx = (T)(xs - ix), fy = (T)(ys - iy), gx = (T)1 - fx, gy = (T)1 - fy;
+        const int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
+        const T fx = (T)(x - ix), fy = (T)(y - iy), gx = (T)1 - fx, gy = (T)1 - fy;
         const int o00 = tile32(ix, iy, Nx);
-        const int o01 = (ix & 3) != 3 ? o00 + 1 : tile32(ix + 1, iy, Nx);
-        const int o10 = (iy & 3) != 3 ? o00 + 4 : tile32(ix, iy + 1, Nx);
-        const int o11 = (ix & 3) != 3 ? o10 + 1 : tile32(ix + 1, iy + 1, Nx);
+        const int sx = (ix & 3) == 3 ? 13 : 1, sy = (iy & 3) == 3 ? rowstep : 4;
+        const int o01 = o00 + sx, o10 = o00 + sy, o11 = o10 + sx;
         // corner values u (mu) and v (dmu/dr) from one or two radius nodes
         T u[4], v[4];
-        if (nn == 1) {  // uniform: first-order Taylor step about the only node
+        if constexpr (ONE) {  // first-order Taylor step about the only node
           const T dr = (T)(r - rlo);
           const cpx<T> c0 = base[o00], c1 = base[o01], c2 = base[o10], c3 = base[o11];
           u[0] = c0.x + dr * c0.y; v[0] = c0.y;
@@ -423,7 +477,10 @@ __global__ __launch_bounds__(256) void k_lc_band(
             wa = 1; wda = dr; wb = 0; wdb = 0;
             za = 0; zda = 1; zb = 0; zdb = 0;
           } else {
-            const T S = (T)((r - ra) / h), S2 = S * S, S3 = S2 * S, H = (T)h, iH = (T)(1.0 / h);
+            // FP32 maps: 1/h from the table; FP64 maps keep the division
+            const double ih = kF32 ? sih[ka] : 1.0 / h;
+            const T S = (T)(kF32 ? (r - ra) * ih : (r - ra) / h), S2 = S * S, S3 = S2 * S;
+            const T H = (T)h, iH = (T)ih;
             wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
             wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
             za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
@@ -443,14 +500,17 @@ __global__ __launch_bounds__(256) void k_lc_band(
         const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
         const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
         const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
-        const double mu = (double)mu_t;
-        flag |= !(in && mu > 0.0 && mu < INFINITY);
-        const double is = isg[j];
-        const double rho = (mobs[j] - fma(kMag, ln_pos(mu), R.dm)) * is;
-        const double q = rho * is;
-        double rmu;
-        if constexpr (sizeof(T) == 4) rmu = (double)__builtin_amdgcn_rcpf(mu_t);
-        else rmu = 1.0 / mu;
+        flag |= !(mu_t > (T)0 && mu_t < (T)INFINITY);
+        double lnmu, rmu;
+        if constexpr (kF32) {
+          lnmu = ln_f32(mu_t, lnt);
+          rmu = (double)__builtin_amdgcn_rcpf(mu_t);
+        } else {
+          lnmu = ln_pos(mu_t);
+          rmu = 1.0 / mu_t;
+        }
+        const double rho = (E.m - fma(kMag, lnmu, R.dm)) * E.is;
+        const double q = rho * E.is;
         const double dc = kDc * q * rmu, tdc = dc * te;
         const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
         s[0] = fma(rho, rho, s[0]);
@@ -674,8 +734,8 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
   const int ibh = ws.add(sizeof(int) * m), ist = ws.add(sizeof(int) * 2 * (P + 1)),
             itot = ws.add(sizeof(int) * 2 * (size_t)nq),
             irec = ws.add(sizeof(CurveRec) * (size_t)n), iacc = ws.add(sizeof(double) * 8 * (size_t)n),
-            ibad = ws.add(sizeof(int) * (size_t)n), iisg = ws.add(sizeof(double) * std::max(a.nepoch, 1)),
-            ipos = ws.add(sizeof(int) * (size_t)n);
+            ibad = ws.add(sizeof(int) * (size_t)n), iep = ws.add(sizeof(Epoch) * std::max(a.nepoch, 1)),
+            irng = ws.add(sizeof(double2) * (size_t)a.nobs * B), ipos = ws.add(sizeof(int) * (size_t)n);
   ws.commit();
   int* bh = ws.get<int>(ibh);
   int *start = ws.get<int>(ist), *cnt = start + (P + 1);
@@ -683,7 +743,8 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
   CurveRec* rec = ws.get<CurveRec>(irec);
   double* acc = ws.get<double>(iacc);
   int* bad = ws.get<int>(ibad);
-  double* isg = ws.get<double>(iisg);
+  Epoch* ep = ws.get<Epoch>(iep);
+  double2* rng = ws.get<double2>(irng);
   int* pos_of = ws.get<int>(ipos);
   const size_t hl = sizeof(int) * nq;
   MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(256), hl, pg, n, bh);
@@ -693,8 +754,8 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
              cnt);
   MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, pg, n, bh, qbase, rec, pos_of);
   if (a.nepoch > 0)
-    MLC_LAUNCH(ctx, "lc_partition", k_inv_sig, dim3(cdiv(a.nepoch, 256)), dim3(256), 0, a.sig, a.nepoch,
-               isg);
+    MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
+               a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
   constexpr int kCurvesPerBlock = 4 * (64 / kLanes);
   // one resident wave per SIMD slot (the kernel runs 4 waves per SIMD)
   const int gb = (int)(cdiv(std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, 4)), 8) * 8);
@@ -708,17 +769,24 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
       bn.bsc = a.band_scale[b];
       for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
       const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
-#define MLC_BAND(TT, F)                                                                              \
-  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F>), dim3(gb), dim3(256), 0, (const cpx<TT>*)base,  \
-             a.Nx, a.Ny, bn, p, b, B, rec, start, cnt, a.nobs, a.obs_off, a.nepoch, a.t, a.mobs, isg, \
-             acc, bad)
+#define MLC_BAND(TT, F, O)                                                                          \
+  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O>), dim3(gb), dim3(256), 0,                    \
+             (const cpx<TT>*)base, a.Nx, a.Ny, bn, p, b, B, rec, start, cnt, a.nobs, a.obs_off,      \
+             a.nepoch, ep, rng, acc, bad)
+#define MLC_BAND_T(TT)                          \
+  if (bn.cnt == 1) {                            \
+    if (b == 0) MLC_BAND(TT, true, true);       \
+    else MLC_BAND(TT, false, true);             \
+  } else {                                      \
+    if (b == 0) MLC_BAND(TT, true, false);      \
+    else MLC_BAND(TT, false, false);            \
+  }
       if (a.precision == MLC_F64) {
-        if (b == 0) MLC_BAND(double, true);
-        else MLC_BAND(double, false);
+        MLC_BAND_T(double)
       } else {
-        if (b == 0) MLC_BAND(float, true);
-        else MLC_BAND(float, false);
+        MLC_BAND_T(float)
       }
+#undef MLC_BAND_T
 #undef MLC_BAND
     }
   MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, P, start, pos_of,

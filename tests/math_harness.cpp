@@ -6,3 +6,9 @@
 extern "C" void h_ln(const double* x, int n, double* out) {
   for (int i = 0; i < n; ++i) out[i] = mlc::ln_pos(x[i]);
 }
+
+extern "C" void h_ln_f32(const float* x, int n, double* out) {
+  mlc::LnEntry tab[mlc::kLnTab];
+  for (int i = 0; i < mlc::kLnTab; ++i) tab[i] = mlc::ln_entry(i);
+  for (int i = 0; i < n; ++i) out[i] = mlc::ln_f32(x[i], tab);
+}

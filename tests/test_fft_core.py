@@ -33,6 +33,7 @@ def fftlib(tmp_path_factory):
 def mathlib(tmp_path_factory):
     L = _build("math_harness.cpp", tmp_path_factory)
     L.h_ln.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.h_ln_f32.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     return L
 
 
@@ -105,3 +106,27 @@ def test_ln_pos_within_two_ulp(mathlib):
     err = np.abs(out - want)
     # two ulp of the result, or 2^-53 absolute around x = 1 where ln x -> 0
     assert np.all(err <= np.maximum(2 * ulp, 1.2e-16 * np.abs(x - 1) + 1e-300)), float(np.max(err / ulp))
+
+
+def test_ln_f32_table_log(mathlib):
+    """The table log the FP32 light-curve kernels use, against numpy's FP64
+    log of the same float: every table interval and both of its ends, the
+    whole float exponent range, subnormals, and values near 1."""
+    rng = np.random.default_rng(1)
+    f = np.arange(1 << 23, dtype=np.uint32)[:: 97]                      # fraction patterns
+    mant = (f | np.uint32(0x3F800000)).view(np.float32)                  # [1, 2)
+    ends = np.array([1.0 + i / 256.0 for i in range(257)], np.float32)
+    x = np.concatenate([
+        mant, ends, np.nextafter(ends, np.float32(0)),
+        np.exp(rng.uniform(-103.0, 88.7, 50000)).astype(np.float32),
+        (1.0 + rng.uniform(-1e-3, 1e-3, 20000)).astype(np.float32),
+        np.array([1.0, 2.0, 0.5, 1e-45, 1.1754942e-38, 1.17549435e-38, 3.4028235e38], np.float32),
+        np.nextafter(np.float32(1.0), np.float32(0)) * np.ones(1, np.float32),
+        np.nextafter(np.float32(1.0), np.float32(2)) * np.ones(1, np.float32),
+    ]).astype(np.float32)
+    x = x[x > 0]
+    out = np.zeros(x.size)
+    mathlib.h_ln_f32(_p(x), x.size, _p(out))
+    want = np.log(x.astype(np.float64))
+    err = np.abs(out - want)
+    assert np.all(err <= 2 * np.spacing(np.abs(want)) + 3e-16), float(np.max(err))

@@ -83,6 +83,40 @@ def test_f64_matches_oracle(ctx, orc, plain):
     assert np.max(np.abs(g[fin] - rg[fin]) / scale) < 1e-10
 
 
+def test_f32_matches_oracle_on_f32_maps(ctx, orc):
+    """The FP32 fused path (single- and multi-node bands) against the FP64
+    oracle run on the same FP32 node maps, with curves that start off the
+    map, leave it part-way, end exactly on the last pixel centre, or meet a
+    NaN epoch time."""
+    rng = np.random.default_rng(16)
+    m, spec, nodes, obs, sets = _setup(ctx, "f32", rng)
+    B, Ny, Nx = nodes.B, 128, 160
+    t_nan = np.array([1.0, np.nan, 7.0])
+    sets.append([(np.array([3.0, 8.0]), np.array([-1.6, -1.8]), np.array([0.05, 0.05])),
+                 (t_nan, np.full(3, -1.7), np.full(3, 0.05))][:B])
+    sets.append([(np.array([2.0, 5.0, 9.0]), np.full(3, -1.7), np.full(3, 0.06)),
+                 (np.array([1.0, 4.0, 12.0]), np.full(3, -1.5), np.full(3, 0.04))][:B])
+    obs = lcm.Observations(ctx, B, sets)
+    prm, prof, obsi = _params(rng, 400, Ny, Nx, nodes.P, 3)
+    prm[0, 0] = -5.0                                   # off the map from the start
+    prm[1, :4] = (Nx - 1 - 10.0, 60.0, 0.0, 1.0)       # leaves through x = Nx - 1
+    obsi[1] = 1
+    prm[2, :4] = (Nx - 1 - 12.0, 50.0, 0.0, 1.0)       # x = Nx - 1 exactly at t = 12
+    obsi[2] = 4
+    obsi[3] = 3                                        # NaN epoch time
+    c, g = lcm.chi2_grad(ctx, nodes, obs, torch.tensor(prm, device="cuda"), torch.tensor(prof, device="cuda"),
+                         torch.tensor(obsi, device="cuda"))
+    rc, rg = _oracle(orc, nodes, sets, prm, prof, obsi)
+    c, g = c.cpu().numpy(), g.cpu().numpy()
+    fin = np.isfinite(rc)
+    assert not fin[0] and not fin[3] and fin[2]
+    assert np.array_equal(np.isfinite(c), fin)
+    assert not g[~fin].any()
+    assert np.max(np.abs(c[fin] - rc[fin]) / np.abs(rc[fin])) < 1e-5
+    scale = np.maximum(np.abs(rg[fin]).max(axis=1, keepdims=True), 1e-30)
+    assert np.max(np.abs(g[fin] - rg[fin]) / scale) < 1e-4
+
+
 def test_adjoint_matches_central_differences(ctx):
     rng = np.random.default_rng(12)
     m, spec, nodes, obs, sets = _setup(ctx, "f64", rng)
