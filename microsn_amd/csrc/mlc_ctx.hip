@@ -3,6 +3,9 @@
 // sampling.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "mlc_internal.hpp"
 
@@ -30,6 +33,20 @@ int fill_grid(mlc_ctx* ctx, int per_cu) {
   hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
             "hipDeviceGetAttribute(CU count)");
   return std::max(1, cus) * per_cu;
+}
+
+int resident_blocks(const void* kernel, int threads) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({kernel, threads});
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  nb = std::max(1, nb);
+  cache[{kernel, threads}] = nb;
+  return nb;
 }
 
 }  // namespace mlc

@@ -168,6 +168,10 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 // Number of workgroups that fill the device a few times over (grid-stride kernels).
 int fill_grid(mlc_ctx* ctx, int per_cu);
 
+// Workgroups of `threads` lanes of `kernel` resident at once on one CU (the
+// occupancy limit of its registers and LDS), at least 1; cached per kernel.
+int resident_blocks(const void* kernel, int threads);
+
 // Turns an exception into a status code and records the message.
 template <class F>
 int guarded(mlc_ctx* ctx, F&& f) {

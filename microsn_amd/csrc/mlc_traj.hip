@@ -425,13 +425,16 @@ __global__ __launch_bounds__(256) void k_lc_band(
     const bool act = i < ncur;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool flag = false;
+    CurveRec R{};
+    int e0 = 0, e1 = 0;
+    bool ok = false;
     if (act) {
-      const CurveRec R = rec[pos0 + i];
+      R = rec[pos0 + i];
       const int o = R.o;
       const bool okobs = o >= 0 && o < nobs;
-      const int e0 = okobs ? obs_off[(size_t)o * B + b] : 0;
-      const int e1 = okobs ? obs_off[(size_t)o * B + b + 1] : 0;
-      bool ok = okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1;
+      e0 = okobs ? obs_off[(size_t)o * B + b] : 0;
+      e1 = okobs ? obs_off[(size_t)o * B + b + 1] : 0;
+      ok = okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1;
       if (ok && e0 < e1) {
         const double2 tr = rng[(size_t)o * B + b];
         const double xa = fma(R.vct, tr.x, R.x0), xb = fma(R.vct, tr.y, R.x0);
@@ -440,88 +443,119 @@ __global__ __launch_bounds__(256) void k_lc_band(
              yb >= 0.0 && yb <= ymax;
       }
       flag = !ok;
-      const int jend = ok ? e1 : e0;
-      int j = e0 + li;
-      Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
-      for (; j < jend; j += kLanes) {
-        const Epoch E = en;
-        if (j + kLanes < jend) en = ep[j + kLanes];
-        const double te = E.t;
-        // inside the hull (checked above): 0 <= x <= Nx - 1, 0 <= y <= Ny - 1
-        const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
-        const double r = bsc * fma(R.w, te, R.r0);
-        const int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
-        const T fx = (T)(x - ix), fy = (T)(y - iy), gx = (T)1 - fx, gy = (T)1 - fy;
-        const int o00 = tile32(ix, iy, Nx);
-        const int sx = (ix & 3) == 3 ? 13 : 1, sy = (iy & 3) == 3 ? rowstep : 4;
-        const int o01 = o00 + sx, o10 = o00 + sy, o11 = o10 + sx;
-        // corner values u (mu) and v (dmu/dr) from one or two radius nodes
-        T u[4], v[4];
-        if constexpr (ONE) {  // first-order Taylor step about the only node
-          const T dr = (T)(r - rlo);
-          const cpx<T> c0 = base[o00], c1 = base[o01], c2 = base[o10], c3 = base[o11];
-          u[0] = c0.x + dr * c0.y; v[0] = c0.y;
-          u[1] = c1.x + dr * c1.y; v[1] = c1.y;
-          u[2] = c2.x + dr * c2.y; v[2] = c2.y;
-          u[3] = c3.x + dr * c3.y; v[3] = c3.y;
-        } else {
-          // node interval k, k+1; Taylor about the end node outside [rlo, rhi]
-          int k = 0;
-          while (k < nn - 2 && srad[k + 1] <= r) ++k;
-          const bool lo = r <= rlo, hi = r >= rhi;
-          const int ka = hi ? nn - 1 : k, kb = hi ? nn - 1 : k + 1;
-          const double ra = srad[ka], h = srad[kb] - ra;
-          T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
-          if (lo || hi) {
-            const T dr = (T)(r - ra);
-            wa = 1; wda = dr; wb = 0; wdb = 0;
-            za = 0; zda = 1; zb = 0; zdb = 0;
-          } else {
-            // FP32 maps: 1/h from the table; FP64 maps keep the division
-            const double ih = kF32 ? sih[ka] : 1.0 / h;
-            const T S = (T)(kF32 ? (r - ra) * ih : (r - ra) / h), S2 = S * S, S3 = S2 * S;
-            const T H = (T)h, iH = (T)ih;
-            wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
-            wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
-            za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
-            zb = (-6 * S2 + 6 * S) * iH; zdb = 3 * S2 - 2 * S;
-          }
-          const cpx<T>* ma = base + mapsz * ka;
-          const cpx<T>* mb = base + mapsz * kb;
-          const int off[4] = {o00, o01, o10, o11};
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const cpx<T> A = ma[off[q]], Bq = mb[off[q]];
-            u[q] = wa * A.x + wda * A.y + wb * Bq.x + wdb * Bq.y;
-            v[q] = za * A.x + zda * A.y + zb * Bq.x + zdb * Bq.y;
-          }
-        }
-        const T mu_t = gy * (gx * u[0] + fx * u[1]) + fy * (gx * u[2] + fx * u[3]);
-        const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
-        const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
-        const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
-        flag |= !(mu_t > (T)0 && mu_t < (T)INFINITY);
-        double lnmu, rmu;
+    }
+    // one epoch of this lane's curve: accumulates into s, raises flag
+    auto sample = [&](const double te, const double em, const double eis) {
+      // inside the hull (checked above): 0 <= x <= Nx - 1, 0 <= y <= Ny - 1
+      const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
+      const double r = bsc * fma(R.w, te, R.r0);
+      const int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
+      const T fx = (T)(x - ix), fy = (T)(y - iy), gx = (T)1 - fx, gy = (T)1 - fy;
+      const int o00 = tile32(ix, iy, Nx);
+      const int sx = (ix & 3) == 3 ? 13 : 1, sy = (iy & 3) == 3 ? rowstep : 4;
+      const int o01 = o00 + sx, o10 = o00 + sy, o11 = o10 + sx;
+      // corner values u (mu) and v (dmu/dr) from one or two radius nodes
+      T u[4], v[4];
+      if constexpr (ONE) {  // first-order Taylor step about the only node
+        const T dr = (T)(r - rlo);
+        // 32-bit byte offsets from the uniform map base (the launcher
+        // checks that the map is below 4 GiB)
+        const char* bb = reinterpret_cast<const char*>(base);
+        auto at = [&](int o) -> const cpx<T>* {
+          return reinterpret_cast<const cpx<T>*>(bb + (uint32_t)o * (uint32_t)sizeof(cpx<T>));
+        };
+        cpx<T> c0, c1, c2, c3;
         if constexpr (kF32) {
-          lnmu = ln_f32(mu_t, lnt);
-          rmu = (double)__builtin_amdgcn_rcpf(mu_t);
+          // the left and right taps of a row are neighbours in the tile
+          // unless ix % 4 == 3: one 16-byte load per row, and a second
+          // tap load only on lanes whose cell crosses a tile column (the
+          // 16-byte load stays inside the map: ix + 1 <= Nx - 1)
+          typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+          const f4a8 top = *reinterpret_cast<const f4a8*>(at(o00));
+          const f4a8 bot = *reinterpret_cast<const f4a8*>(at(o10));
+          c0 = cpx<T>{top.x, top.y};
+          c2 = cpx<T>{bot.x, bot.y};
+          if (sx == 1) {
+            c1 = cpx<T>{top.z, top.w};
+            c3 = cpx<T>{bot.z, bot.w};
+          } else {
+            c1 = *at(o01);
+            c3 = *at(o11);
+          }
         } else {
-          lnmu = ln_pos(mu_t);
-          rmu = 1.0 / mu_t;
+          c0 = *at(o00); c1 = *at(o01); c2 = *at(o10); c3 = *at(o11);
         }
-        const double rho = (E.m - fma(kMag, lnmu, R.dm)) * E.is;
-        const double q = rho * E.is;
-        const double dc = kDc * q * rmu, tdc = dc * te;
-        const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
-        s[0] = fma(rho, rho, s[0]);
-        s[1] = fma(dc, mx, s[1]);
-        s[2] = fma(dc, my, s[2]);
-        s[3] = fma(tdc, mx, s[3]);
-        s[4] = fma(tdc, my, s[4]);
-        s[5] = fma(dc, mr, s[5]);
-        s[6] = fma(tdc, mr, s[6]);
-        s[7] += q;
+        u[0] = c0.x + dr * c0.y; v[0] = c0.y;
+        u[1] = c1.x + dr * c1.y; v[1] = c1.y;
+        u[2] = c2.x + dr * c2.y; v[2] = c2.y;
+        u[3] = c3.x + dr * c3.y; v[3] = c3.y;
+      } else {
+        // node interval k, k+1; Taylor about the end node outside [rlo, rhi]
+        int k = 0;
+        while (k < nn - 2 && srad[k + 1] <= r) ++k;
+        const bool lo = r <= rlo, hi = r >= rhi;
+        const int ka = hi ? nn - 1 : k, kb = hi ? nn - 1 : k + 1;
+        const double ra = srad[ka], h = srad[kb] - ra;
+        T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
+        if (lo || hi) {
+          const T dr = (T)(r - ra);
+          wa = 1; wda = dr; wb = 0; wdb = 0;
+          za = 0; zda = 1; zb = 0; zdb = 0;
+        } else {
+          // FP32 maps: 1/h from the table; FP64 maps keep the division
+          const double ih = kF32 ? sih[ka] : 1.0 / h;
+          const T S = (T)(kF32 ? (r - ra) * ih : (r - ra) / h), S2 = S * S, S3 = S2 * S;
+          const T H = (T)h, iH = (T)ih;
+          wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
+          wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
+          za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
+          zb = (-6 * S2 + 6 * S) * iH; zdb = 3 * S2 - 2 * S;
+        }
+        const cpx<T>* ma = base + mapsz * ka;
+        const cpx<T>* mb = base + mapsz * kb;
+        const int off[4] = {o00, o01, o10, o11};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const cpx<T> A = ma[off[q]], Bq = mb[off[q]];
+          u[q] = wa * A.x + wda * A.y + wb * Bq.x + wdb * Bq.y;
+          v[q] = za * A.x + zda * A.y + zb * Bq.x + zdb * Bq.y;
+        }
       }
+      const T mu_t = gy * (gx * u[0] + fx * u[1]) + fy * (gx * u[2] + fx * u[3]);
+      const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
+      const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
+      const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
+      flag |= !(mu_t > (T)0 && mu_t < (T)INFINITY);
+      double lnmu, rmu;
+      if constexpr (kF32) {
+        lnmu = ln_f32(mu_t, lnt);
+        rmu = (double)__builtin_amdgcn_rcpf(mu_t);
+      } else {
+        lnmu = ln_pos(mu_t);
+        rmu = 1.0 / mu_t;
+      }
+      const double rho = (em - fma(kMag, lnmu, R.dm)) * eis;
+      const double q = rho * eis;
+      const double dc = kDc * q * rmu, tdc = dc * te;
+      const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
+      s[0] = fma(rho, rho, s[0]);
+      s[1] = fma(dc, mx, s[1]);
+      s[2] = fma(dc, my, s[2]);
+      s[3] = fma(tdc, mx, s[3]);
+      s[4] = fma(tdc, my, s[4]);
+      s[5] = fma(dc, mr, s[5]);
+      s[6] = fma(tdc, mr, s[6]);
+      s[7] += q;
+    };
+    // epochs of this lane: j = e0 + li + kLanes k, each record read one
+    // iteration ahead
+    const int jend = ok ? e1 : e0;
+    int j = e0 + li;
+    Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
+    for (; j < jend; j += kLanes) {
+      const Epoch E = en;
+      if (j + kLanes < jend) en = ep[j + kLanes];
+      sample(E.t, E.m, E.is);
     }
     const double val = group_sum8<kLanes>(s, li);
     const unsigned long long bm = __ballot(flag);
@@ -757,8 +791,12 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
     MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
                a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
   constexpr int kCurvesPerBlock = 4 * (64 / kLanes);
-  // one resident wave per SIMD slot (the kernel runs 4 waves per SIMD)
-  const int gb = (int)(cdiv(std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, 4)), 8) * 8);
+  // as many workgroups as the CUs hold at once (the kernel's occupancy), a
+  // multiple of 8 so that every XCD gets the same number
+  auto grid_for = [&](const void* k) {
+    return (int)(cdiv(std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, resident_blocks(k, 256))), 8) *
+                 8);
+  };
   const size_t mapsz = (size_t)a.Nx * a.Ny;
   const size_t esz = a.precision == MLC_F64 ? 2 * sizeof(double) : 2 * sizeof(float);
   for (int p = 0; p < P; ++p)
@@ -770,11 +808,12 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
       for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
       const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
 #define MLC_BAND(TT, F, O)                                                                          \
-  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O>), dim3(gb), dim3(256), 0,                    \
+  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O>),                                            \
+             dim3(grid_for(reinterpret_cast<const void*>(&k_lc_band<TT, F, O>))), dim3(256), 0,      \
              (const cpx<TT>*)base, a.Nx, a.Ny, bn, p, b, B, rec, start, cnt, a.nobs, a.obs_off,      \
              a.nepoch, ep, rng, acc, bad)
 #define MLC_BAND_T(TT)                          \
-  if (bn.cnt == 1) {                            \
+  if (bn.cnt == 1 && mapsz * 2 * sizeof(TT) < ((size_t)1 << 32)) { \
     if (b == 0) MLC_BAND(TT, true, true);       \
     else MLC_BAND(TT, false, true);             \
   } else {                                      \
