@@ -124,6 +124,14 @@ int mlc_profile_size(int shape, double rhalf, double pix, double rin, double a);
  * At least one of them must be given. */
 int mlc_map_convolve(mlc_map* map, int shape, double rhalf, double pix,
                      double rin, double a, void* d_value, void* d_node);
+/* Node maps of count profiles in one call: d_nodes[k] receives what
+ * mlc_map_convolve(map, shape[k], rhalf[k], pix, rin[k], a[k], NULL,
+ * d_nodes[k]) writes, bit for bit (rin and a may be NULL: 0 and 1).  The
+ * profiles and their cosine taps of all k are built in one launch each; the
+ * per-profile loop of expanding_source (gpu_functions.cu:180-210) over the
+ * source sizes is the caller this serves.  Host arrays, device d_nodes[k]. */
+int mlc_map_convolve_nodes(mlc_map* map, int count, const int* shape, const double* rhalf,
+                           double pix, const double* rin, const double* a, void* const* d_nodes);
 /* The same computation structured like the reference (plain path): build
  * the full Nx x Ny kernel array on the device, forward 2-D FFT, separate
  * multiply launch, inverse 2-D FFT, separate interleave, stream-synchronised

@@ -141,6 +141,25 @@ int mlc_map_convolve(mlc_map* m, int shape, double rhalf, double pix, double rin
   });
 }
 
+int mlc_map_convolve_nodes(mlc_map* m, int count, const int* shape, const double* rhalf, double pix,
+                           const double* rin, const double* a, void* const* d_nodes) {
+  if (!m) return MLC_ERR_ARG;
+  return guarded(m->ctx, [&] {
+    MLC_REQUIRE(count >= 0, "mlc_map_convolve_nodes: negative count");
+    if (count == 0) return;
+    MLC_REQUIRE(shape && rhalf && d_nodes, "mlc_map_convolve_nodes: null pointer");
+    MLC_REQUIRE(m->Nx % 4 == 0 && m->Ny % 4 == 0,
+                "mlc_map_convolve_nodes: node maps (4 x 4 tiles) need Nx and Ny divisible by 4");
+    std::vector<ProfileParams> Ps(count);
+    for (int k = 0; k < count; ++k) {
+      MLC_REQUIRE(d_nodes[k], "mlc_map_convolve_nodes: null node map");
+      Ps[k] = make_params(shape[k], rhalf[k], pix, rin ? rin[k] : 0.0, a ? a[k] : 1.0, m->Nx, m->Ny);
+    }
+    if (m->precision == MLC_F64) fft_convolve_nodes<double>(m, Ps, d_nodes);
+    else fft_convolve_nodes<float>(m, Ps, d_nodes);
+  });
+}
+
 int mlc_map_convolve_plain(mlc_map* m, int shape, double rhalf, double pix, double rin, double a,
                            void* d_value, void* d_node) {
   if (!m) return MLC_ERR_ARG;

@@ -3,6 +3,8 @@
 // (mlc_fft_f32.hip, mlc_fft_f64.hip).
 #pragma once
 
+#include <vector>
+
 #include "mlc_fft_core.hpp"
 #include "mlc_internal.hpp"
 #include "mlc_profile.hpp"
@@ -35,6 +37,9 @@ void fft_map_transform(mlc_map* m, const double* d_map);
 // Fused convolution (see mlc_map_convolve).
 template <class T>
 void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
+// Node maps of several profiles (see mlc_map_convolve_nodes).
+template <class T>
+void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes);
 // Reference-structured convolution (see mlc_map_convolve_plain).
 template <class T>
 void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
@@ -48,6 +53,8 @@ extern template void fft_map_transform<float>(mlc_map*, const double*);
 extern template void fft_map_transform<double>(mlc_map*, const double*);
 extern template void fft_convolve_fused<float>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_convolve_fused<double>(mlc_map*, const ProfileParams&, void*, void*);
+extern template void fft_convolve_nodes<float>(mlc_map*, const std::vector<ProfileParams>&, void* const*);
+extern template void fft_convolve_nodes<double>(mlc_map*, const std::vector<ProfileParams>&, void* const*);
 extern template void fft_convolve_plain<float>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_convolve_plain<double>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_r2c_2d<float>(mlc_ctx*, const float*, cpx<float>*, int, int);

@@ -482,8 +482,8 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
 // ------------------------------------------------------------------ profiles
 // One workgroup evaluates the n x n profile and its d/drhalf, normalises both
 // (K = f/S, dK = (df - K dS)/S) in FP64.
-__global__ __launch_bounds__(1024) void k_profile(ProfileParams P, double* __restrict__ prof,
-                                                  double* __restrict__ dprof) {
+__device__ __forceinline__ void profile_block(const ProfileParams& P, double* __restrict__ prof,
+                                              double* __restrict__ dprof) {
   __shared__ double red[2][1024];
   const int n = P.n, t = threadIdx.x;
   double s = 0.0, ds = 0.0;
@@ -515,16 +515,35 @@ __global__ __launch_bounds__(1024) void k_profile(ProfileParams P, double* __res
   }
 }
 
+__global__ __launch_bounds__(1024) void k_profile(ProfileParams P, double* __restrict__ prof,
+                                                  double* __restrict__ dprof) {
+  profile_block(P, prof, dprof);
+}
+
+// Several node maps in one call: profile k at prof + off_p, its cosine
+// coefficients at av / ad + off_a (offsets in doubles).
+struct NodeJob {
+  ProfileParams P;
+  int64_t off_p, off_a;
+};
+
+// One workgroup per job.
+__global__ __launch_bounds__(1024) void k_profile_many(const NodeJob* __restrict__ jobs,
+                                                       double* __restrict__ prof,
+                                                       double* __restrict__ dprof) {
+  const NodeJob J = jobs[blockIdx.x];
+  profile_block(J.P, prof + J.off_p, dprof + J.off_p);
+}
+
 // Cosine coefficients of the symmetric kernel (see k_node_col):
 //   a[k][u] = rho[h+k][u] + rho[h-1-k][u],
 //   rho[s][u] = sum_c P[s][c] cos(pi u (2(c-h)+1)/Nx),
 // from the table cosx[m] = cos(pi m/Nx), m < 2 Nx.  FP64.  grid (ceil(W/256), h).
-__global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ prof,
-                                                    const double* __restrict__ dprof, int n, int Nx,
-                                                    int W, const double* __restrict__ cosx,
-                                                    double* __restrict__ av, double* __restrict__ ad) {
-  const int u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, h = n >> 1;
-  if (u >= W) return;
+__device__ __forceinline__ void kernel_cos_at(const double* __restrict__ prof,
+                                              const double* __restrict__ dprof, int n, int Nx, int W,
+                                              const double* __restrict__ cosx, double* __restrict__ av,
+                                              double* __restrict__ ad, int u, int k) {
+  const int h = n >> 1;
   const int s1 = h + k, s2 = h - 1 - k, twoN = 2 * Nx;
   double sa = 0.0, sd = 0.0;
   // angle index u (2(cc-h)+1) mod 2Nx, advanced by 2u per column
@@ -539,6 +558,28 @@ __global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ p
   }
   av[(size_t)k * W + u] = sa;
   if (ad) ad[(size_t)k * W + u] = sd;
+}
+
+__global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ prof,
+                                                    const double* __restrict__ dprof, int n, int Nx,
+                                                    int W, const double* __restrict__ cosx,
+                                                    double* __restrict__ av, double* __restrict__ ad) {
+  const int u = blockIdx.x * 256 + threadIdx.x;
+  if (u >= W) return;
+  kernel_cos_at(prof, dprof, n, Nx, W, cosx, av, ad, u, blockIdx.y);
+}
+
+// grid (ceil(W/256), max h, jobs).
+__global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restrict__ jobs,
+                                                         const double* __restrict__ prof,
+                                                         const double* __restrict__ dprof, int Nx,
+                                                         int W, const double* __restrict__ cosx,
+                                                         double* __restrict__ av,
+                                                         double* __restrict__ ad) {
+  const NodeJob& J = jobs[blockIdx.z];
+  const int n = J.P.n, u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y;
+  if (u >= W || k >= (n >> 1)) return;
+  kernel_cos_at(prof + J.off_p, dprof + J.off_p, n, Nx, W, cosx, av + J.off_a, ad + J.off_a, u, k);
 }
 
 // Full Ny x Nx wrapped kernel (Kernel::setKernel), for the plain path.
@@ -938,6 +979,50 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
   if (d_value) row_c2r<T, 1>(ctx, G0, nullptr, (T*)d_value, m->Nx, m->Ny, m->mean);
 }
 
+// Node maps of several profiles: the profiles and their cosine
+// coefficients in one launch each, then the y-filter and row transforms of
+// each node in turn (the same kernels and arithmetic as fft_convolve_fused).
+template <class T>
+void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes) {
+  mlc_ctx* ctx = m->ctx;
+  const int cnt = (int)Ps.size(), W = m->W;
+  if (cnt == 0) return;
+  std::vector<NodeJob> jobs(cnt);
+  int64_t np = 0, na = 0;
+  int hmax = 1;
+  for (int k = 0; k < cnt; ++k) {
+    const int n = Ps[k].n, h = n / 2;
+    jobs[k] = NodeJob{Ps[k], np, na};
+    np += (int64_t)n * n;
+    na += (int64_t)h * W;
+    hmax = std::max(hmax, h);
+  }
+  Workspace ws(ctx);
+  int ij = ws.add(sizeof(NodeJob) * cnt);
+  int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
+  int ia = ws.add(sizeof(double) * std::max<int64_t>(na, 1)), iad = ws.add(sizeof(double) * std::max<int64_t>(na, 1));
+  ws.commit();
+  NodeJob* dj = ws.get<NodeJob>(ij);
+  double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
+  double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
+  h2d(ctx, dj, jobs.data(), sizeof(NodeJob) * cnt);
+  MLC_LAUNCH(ctx, "profile", k_profile_many, dim3(cnt), dim3(1024), 0, dj, prof, dprof);
+  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many, dim3(cdiv(W, 256), hmax, cnt), dim3(256), 0, dj,
+             prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad);
+  cpx<T>* G0 = (cpx<T>*)m->g0;
+  cpx<T>* G1 = (cpx<T>*)m->g1;
+  for (int k = 0; k < cnt; ++k) {
+    const int h = Ps[k].n / 2;
+    const double *a = av + jobs[k].off_a, *b = ad + jobs[k].off_a;
+    if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, a, b, h, G0, G1, m->Nx, m->Ny, W)) {
+      ensure_full<T>(m);
+      node_col<T>(ctx, (const cpx<T>*)m->fmap, a, b, h, G0, G1, m->Nx, m->Ny, W);
+    }
+    if (!node_rows<T>(ctx, G0, G1, (T*)d_nodes[k], m->Nx, m->Ny, m->mean))
+      row_c2r<T, 2>(ctx, G0, G1, (T*)d_nodes[k], m->Nx, m->Ny, m->mean);
+  }
+}
+
 template <class T>
 void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node) {
   // The reference's structure (gpu_functions.cu:184-208): full-size kernel,
@@ -998,6 +1083,7 @@ void fft_c2r_2d(mlc_ctx* ctx, cpx<T>* work, T* out, int Nx, int Ny) {
 #define MLC_FFT_INSTANTIATE(T)                                                               \
   template void fft_map_transform<T>(mlc_map*, const double*);                               \
   template void fft_convolve_fused<T>(mlc_map*, const ProfileParams&, void*, void*);         \
+  template void fft_convolve_nodes<T>(mlc_map*, const std::vector<ProfileParams>&, void* const*); \
   template void fft_convolve_plain<T>(mlc_map*, const ProfileParams&, void*, void*);         \
   template void fft_r2c_2d<T>(mlc_ctx*, const T*, cpx<T>*, int, int);                        \
   template void fft_c2r_2d<T>(mlc_ctx*, cpx<T>*, T*, int, int);

@@ -62,10 +62,23 @@ class NodeSet:
         return untile(self.nodes)
 
     def compute(self, plain=False):
-        """(Re)convolve every node map from the spectrum's current map."""
-        f = lib().mlc_map_convolve_plain if plain else lib().mlc_map_convolve
-        for k, (s, r) in enumerate(zip(self.node_shape, self.node_rad)):
-            self.ctx.check(f(self.spectrum.h, s, float(r), self.pix, 0.0, 1.0, None, ptr(self.nodes[k])))
+        """(Re)convolve every node map from the spectrum's current map: one
+        mlc_map_convolve_nodes call, or one mlc_map_convolve_plain call per
+        node on the plain path."""
+        K = len(self.node_rad)
+        if plain:
+            for k, (s, r) in enumerate(zip(self.node_shape, self.node_rad)):
+                self.ctx.check(lib().mlc_map_convolve_plain(self.spectrum.h, s, float(r), self.pix, 0.0, 1.0,
+                                                            None, ptr(self.nodes[k])))
+            return
+        if not hasattr(self, "_batch"):
+            shp = np.ascontiguousarray(self.node_shape, np.int32)
+            rad = np.ascontiguousarray(self.node_rad, np.float64)
+            dst = (C.c_void_p * K)(*[self.nodes[k].data_ptr() for k in range(K)])
+            self._batch = (shp, rad, dst)
+        shp, rad, dst = self._batch
+        self.ctx.check(lib().mlc_map_convolve_nodes(self.spectrum.h, K, shp.ctypes.data, rad.ctypes.data,
+                                                    self.pix, None, None, C.addressof(dst)))
 
 
 def _shape_id(s):

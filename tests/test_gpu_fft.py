@@ -144,3 +144,35 @@ def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec
     nd = node.cpu().numpy().astype(np.float64)
     assert _rel(nd[..., 0], ref) < tol
     assert np.max(np.abs(nd[..., 1] - dref)) < tol * 20 * max(np.max(np.abs(ref)), 1.0)
+
+
+@pytest.mark.parametrize("Ny,Nx", [(128, 160), (128, 256)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_convolve_nodes_equals_one_by_one(ctx, Ny, Nx, prec):
+    """mlc_map_convolve_nodes writes, bit for bit, what one mlc_map_convolve
+    call per node writes: mixed shapes, radii and tap counts in one batch
+    (at 128 x 256 in FP64 the 80-pixel Gaussian takes the column-FFT
+    fallback)."""
+    import ctypes as C
+
+    from microsn_amd._lib import SHAPES, lib
+    from microsn_amd.lightcurves import NodeSet
+
+    rng = np.random.default_rng(Ny + Nx)
+    m = _smooth_map(rng, Ny, Nx)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
+    shapes = ["gaussian", "uniform", "exponential", "gaussian", "wavy"]
+    rads = [2.0, 3.5, 1.5, 20.0 if Nx == 256 else 6.0, 2.5]
+    ns = NodeSet(ctx, spec, shapes, [[[r]] for r in rads], [1.0])  # batched
+    for k, (s, r) in enumerate(zip(shapes, rads)):
+        one = torch.empty_like(ns.nodes[k])
+        ctx.check(lib().mlc_map_convolve(spec.h, SHAPES[s], r, 1.0, 0.0, 1.0, None, one.data_ptr()))
+        assert torch.equal(one, ns.nodes[k]), (s, r)
+    # an empty batch is a no-op; a null node pointer is rejected before any work
+    ctx_h = spec.h
+    assert lib().mlc_map_convolve_nodes(ctx_h, 0, None, None, 1.0, None, None, None) == 0
+    shp = np.array([0], np.int32)
+    rad = np.array([2.0])
+    dst = (C.c_void_p * 1)(None)
+    assert lib().mlc_map_convolve_nodes(ctx_h, 1, shp.ctypes.data, rad.ctypes.data, 1.0, None, None,
+                                        C.addressof(dst)) != 0
