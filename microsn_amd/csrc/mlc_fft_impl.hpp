@@ -103,22 +103,49 @@ __device__ __forceinline__ int wrap_row(int y, int h, int N) {
 // LDS of a row kernel: [twiddles W_M^m, m < tw_len(M)] [row buffer(s)].
 // Forward real FFT of one row per workgroup.  in: Ny x Nx of TIN; out: Ny x W.
 // With shift != NULL the FP64 value *shift is subtracted before rounding to T.
+// With rowsum != NULL the row's own FP64 mean is subtracted instead and its
+// FP64 sum goes to rowsum[y] (k_row_dc_fix then sets the DC terms).
 template <class T, class TIN, int G>
 __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T>* __restrict__ out,
-                                               int Nx, const double* __restrict__ shift, Plan plan,
+                                               int Nx, const double* __restrict__ shift,
+                                               double* __restrict__ rowsum, Plan plan,
                                                const cpx<T>* __restrict__ twM,
                                                const cpx<T>* __restrict__ twN) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[G / 64];
   const int y = blockIdx.x, g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
   cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* buf = q + QL;
   const NoPad P;
   for (int i = g; i < QL; i += G) q[i] = twM[i];
   const cpx<TIN>* row = reinterpret_cast<const cpx<TIN>*>(in + (size_t)y * Nx);
-  const double sh = shift ? *shift : 0.0;
-  for (int n = g; n < M; n += G) {
-    cpx<TIN> p = row[n];
-    buf[P(n)] = mk<T>((T)((double)p.x - sh), (T)((double)p.y - sh));
+  double sh = shift ? *shift : 0.0;
+  // the row in registers (M <= 16 G for every row group)
+  constexpr int kPer = 16;
+  cpx<TIN> v[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int n = g + i * G;
+    if (n < M) v[i] = row[n];
+  }
+  if (rowsum) {
+    double a = 0.0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+      if (g + i * G < M) a += (double)v[i].x + (double)v[i].y;
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if ((g & 63) == 0) red[g >> 6] = a;
+    __syncthreads();
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < G / 64; ++w) tot += red[w];
+    if (g == 0) rowsum[y] = tot;
+    sh = tot / (double)Nx;
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int n = g + i * G;
+    if (n < M) buf[P(n)] = mk<T>((T)((double)v[i].x - sh), (T)((double)v[i].y - sh));
   }
   __syncthreads();
   dev_fft<T, G, 16, false>(buf, plan, QuarterTw<T>{q, QL}, P, g);
@@ -129,6 +156,27 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
     o[k] = Xk;
     if (M - k != k) o[M - k] = Xm;
   }
+}
+
+// After k_row_r2c with row sums: the map mean (sum of the row sums in a
+// fixed order / (Nx Ny)) into *mean and the DC term of every row spectrum,
+// rowsum[y] - Nx mean, the FFT of the row minus the map mean would give.
+template <class T>
+__global__ __launch_bounds__(1024) void k_row_dc_fix(const double* __restrict__ rowsum, int Nx, int Ny,
+                                                     cpx<T>* __restrict__ out, double* __restrict__ mean) {
+  __shared__ double red[1024];
+  const int t = threadIdx.x, W = Nx / 2 + 1;
+  double a = 0.0;
+  for (int y = t; y < Ny; y += 1024) a += rowsum[y];
+  red[t] = a;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  const double mu = red[0] / ((double)Nx * (double)Ny);
+  if (t == 0) *mean = mu;
+  for (int y = t; y < Ny; y += 1024) out[(size_t)y * W] = mk<T>((T)(rowsum[y] - (double)Nx * mu), (T)0);
 }
 
 // Loads the folded inverse input of one row into buf (see c2r_pre).
@@ -613,37 +661,6 @@ __global__ void k_interleave(const T* __restrict__ a, const T* __restrict__ b, c
     o[tile_offset((int)(i % Nx), (int)(i / Nx), Nx)] = mk<T>(a[i], b[i]);
 }
 
-// FP64 sum of a map: per-workgroup partials, then one workgroup adds them in
-// order (deterministic).  out[0] = sum / npix.
-__global__ __launch_bounds__(256) void k_sum_partial(const double* __restrict__ x, int64_t n,
-                                                     double* __restrict__ part) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-    s += x[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ part, int np,
-                                                   double inv_n, double* __restrict__ out) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = red[0] * inv_n;
-}
-
 // ------------------------------------------------------------------ host side
 template <class T>
 const cpx<T>* twiddles(mlc_ctx* ctx, int L) {
@@ -727,7 +744,8 @@ size_t row_lds(int M, int nbuf) {
 }
 
 template <class T, class TIN>
-void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const double* shift) {
+void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const double* shift,
+             double* rowsum = nullptr) {
   const int M = Nx / 2;
   Plan p = plan_for(M, "row FFT", 8);
   const cpx<T>* twM = twiddles<T>(ctx, M);
@@ -738,7 +756,7 @@ void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const dou
     constexpr int G = decltype(gc)::value;
     auto k = k_row_r2c<T, TIN, G>;
     allow_lds(k, lds);
-    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, Nx, shift, p, twM, twN);
+    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, Nx, shift, rowsum, p, twM, twN);
   });
 }
 
@@ -922,20 +940,21 @@ namespace mlc {
 template <class T>
 void fft_map_transform(mlc_map* m, const double* d_map) {
   mlc_ctx* ctx = m->ctx;
-  const int64_t npix = (int64_t)m->Nx * m->Ny;
   if (m->subtract_mean) {
+    // each row is transformed minus its own mean (the sums come out of the
+    // same pass over the map); the map mean and the DC terms follow from the
+    // row sums
     Workspace ws(ctx);
-    int np = 1024;
-    int ip = ws.add(sizeof(double) * np);
+    int ir = ws.add(sizeof(double) * m->Ny);
     ws.commit();
-    MLC_LAUNCH(ctx, "map_sum_partial", k_sum_partial, dim3(np), dim3(256), 0, d_map, npix,
-               ws.get<double>(ip));
-    MLC_LAUNCH(ctx, "map_sum_final", k_sum_final, dim3(1), dim3(256), 0, ws.get<double>(ip), np,
-               1.0 / (double)npix, m->mean);
+    double* rowsum = ws.get<double>(ir);
+    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, rowsum);
+    MLC_LAUNCH(ctx, "map_dc", k_row_dc_fix<T>, dim3(1), dim3(1024), 0, rowsum, m->Nx, m->Ny,
+               (cpx<T>*)m->rows, m->mean);
   } else {
     hip_check(hipMemsetAsync(m->mean, 0, sizeof(double), ctx->stream), "hipMemsetAsync(mean)");
+    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, m->mean);
   }
-  row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, m->mean);
   m->full_ready = false;
 }
 
