@@ -52,6 +52,8 @@ typedef struct mlc_map mlc_map;
  * __constant__ tables gpu_functions.cu:25-33, cuda_error util.cu:34-68). */
 int mlc_version(void);
 int mlc_ctx_create(int device, void* stream, mlc_ctx** out);
+/* Frees the context's own device memory.  Maps created on it (mlc_map_create)
+ * must be destroyed first: a map keeps a pointer to its context. */
 void mlc_ctx_destroy(mlc_ctx* ctx);
 const char* mlc_last_error(const mlc_ctx* ctx);
 int mlc_ctx_set_stream(mlc_ctx* ctx, void* stream);

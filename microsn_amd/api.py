@@ -7,6 +7,7 @@ layouts; device arrays are torch tensors on the context's device.  torch
 provides device memory and the stream; every computation runs in libmlc.so.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 import torch
@@ -45,6 +46,7 @@ class Context:
             raise MlcError(rc, "mlc_ctx_create failed")
         self.h = h
         self.stream = stream
+        self._maps = weakref.WeakSet()  # closed before the context (they point to it)
 
     def check(self, rc):
         if rc != 0:
@@ -55,6 +57,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for m in list(getattr(self, "_maps", ())):
+                m.close()
             lib().mlc_ctx_destroy(self.h)
             self.h = None
 
@@ -115,6 +119,7 @@ class MapSpectrum:
         ctx.check(lib().mlc_map_create(ctx.h, self.prec, ptr(dmap), self.Nx, self.Ny,
                                        int(bool(subtract_mean)), C.byref(h)))
         self.h = h
+        ctx._maps.add(self)
 
     @property
     def dtype(self):

@@ -196,7 +196,9 @@ __device__ __forceinline__ void c2r_fill(cpx<T>* buf, const cpx<T>* __restrict__
 // MODE 0: out is T[Ny][Nx] = x;  MODE 1: out is T[Ny][Nx] = x + *shift;
 // MODE 2: out is the node map (x0 + *shift, x1) of the spectra in0, in1 in
 // 4 x 4 pixel tiles (mlc_tiles.hpp); rows are numbered XCD-contiguously so
-// the four rows of a tile are written through one L2.
+// the four rows of a tile are written through one L2.  MODE 3 / MODE 4:
+// only the first (x0 + *shift) / second (x1) member of every node pair, from
+// in0 alone, for rows too long to hold two buffers in LDS.
 template <class T, int G, int MODE>
 __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
                                                const cpx<T>* __restrict__ in1, T* __restrict__ out,
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
                                                Plan plan, const cpx<T>* __restrict__ twM,
                                                const cpx<T>* __restrict__ twN) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int y = MODE == 2 ? xcd_block(Ny) : (int)blockIdx.x;
+  const int y = MODE >= 2 ? xcd_block(Ny) : (int)blockIdx.x;
   if (y >= Ny) return;
   const int g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
   cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
@@ -227,6 +229,19 @@ __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
       cpx<T>* dst = o + tile_offset(2 * n, y, Nx);
       dst[0] = mk<T>(a.x + sh, b.x);
       dst[1] = mk<T>(a.y + sh, b.y);
+    }
+  } else if constexpr (MODE == 3 || MODE == 4) {
+    cpx<T>* o = reinterpret_cast<cpx<T>*>(out);
+    for (int n = g; n < M; n += G) {
+      cpx<T> a = buf0[P(n)];
+      cpx<T>* dst = o + tile_offset(2 * n, y, Nx);
+      if constexpr (MODE == 3) {
+        dst[0].x = a.x + sh;
+        dst[1].x = a.y + sh;
+      } else {
+        dst[0].y = a.x;
+        dst[1].y = a.y;
+      }
     }
   } else {
     cpx<T>* o = reinterpret_cast<cpx<T>*>(out + (size_t)y * Nx);
@@ -767,14 +782,21 @@ void row_c2r(mlc_ctx* ctx, const cpx<T>* in0, const cpx<T>* in1, T* out, int Nx,
   Plan p = plan_for(M, "row FFT", 8);
   const cpx<T>* twM = twiddles<T>(ctx, M);
   const cpx<T>* twN = twiddles<T>(ctx, Nx);
+  if constexpr (MODE == 2) {
+    if (row_lds<T>(M, 2) > kLdsBytes) {  // one member of the pairs per pass
+      row_c2r<T, 3>(ctx, in0, nullptr, out, Nx, Ny, shift);
+      row_c2r<T, 4>(ctx, in1, nullptr, out, Nx, Ny, nullptr);
+      return;
+    }
+  }
   const size_t lds = row_lds<T>(M, MODE == 2 ? 2 : 1);
   if (lds > kLdsBytes) fail(MLC_ERR_UNSUPPORTED, "row FFT: row does not fit in LDS");
   with_row_group(M, "row FFT", [&](auto gc) {
     constexpr int G = decltype(gc)::value;
     auto k = k_row_c2r<T, G, MODE>;
     allow_lds(k, lds);
-    const int grid = MODE == 2 ? ((Ny + 7) / 8) * 8 : Ny;
-    MLC_LAUNCH(ctx, MODE == 2 ? "fft_row_c2r_node" : "fft_row_c2r", k, dim3(grid), dim3(G), lds, in0,
+    const int grid = MODE >= 2 ? ((Ny + 7) / 8) * 8 : Ny;
+    MLC_LAUNCH(ctx, MODE >= 2 ? "fft_row_c2r_node" : "fft_row_c2r", k, dim3(grid), dim3(G), lds, in0,
                in1, out, Nx, Ny, shift, p, twM, twN);
   });
 }

@@ -138,7 +138,8 @@ struct alignas(64) CurveRec {
 };
 static_assert(sizeof(CurveRec) == 64, "one record per 64 bytes");
 
-constexpr int kPartChunk = 2048;    // curves per partition block (256 threads x 8)
+constexpr int kPartChunk = 2048;    // curves per partition block
+constexpr int kPartThreads = 1024;  // threads of a partition block (2 curves each)
 constexpr int kMaxBuckets = 8192;   // partition bucket counters live in LDS
 constexpr int kMaxBandNodes = 16;   // radius nodes of one (profile, band), kernel argument
 #ifndef MLC_LC_LANES
@@ -184,15 +185,15 @@ __device__ __forceinline__ int curve_bucket(const PartGeom& g, int c) {
 }
 
 // Per-block bucket counts, stored block-major: bh[block * nq + q].
-__global__ __launch_bounds__(256) void k_part_count(PartGeom g, int n, int* __restrict__ bh) {
+__global__ __launch_bounds__(kPartThreads) void k_part_count(PartGeom g, int n, int* __restrict__ bh) {
   extern __shared__ int h[];
   const int nq = g.P * g.C + 1;
-  for (int q = threadIdx.x; q < nq; q += 256) h[q] = 0;
+  for (int q = threadIdx.x; q < nq; q += kPartThreads) h[q] = 0;
   __syncthreads();
   const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
-  for (int c = c0 + threadIdx.x; c < c1; c += 256) atomicAdd(&h[curve_bucket(g, c)], 1);
+  for (int c = c0 + threadIdx.x; c < c1; c += kPartThreads) atomicAdd(&h[curve_bucket(g, c)], 1);
   __syncthreads();
-  for (int q = threadIdx.x; q < nq; q += 256) bh[(size_t)blockIdx.x * nq + q] = h[q];
+  for (int q = threadIdx.x; q < nq; q += kPartThreads) bh[(size_t)blockIdx.x * nq + q] = h[q];
 }
 
 // Exclusive prefix of every bucket's counts over the partition blocks, in
@@ -258,16 +259,16 @@ __global__ __launch_bounds__(1024) void k_part_base(const int* __restrict__ tot,
 // Records of the block's curves at the block's bucket offsets (the order
 // inside one block's share of a bucket is whatever the LDS counter gives; no
 // result depends on it).
-__global__ __launch_bounds__(256) void k_part_fill(PartGeom g, int n, const int* __restrict__ bh,
+__global__ __launch_bounds__(kPartThreads) void k_part_fill(PartGeom g, int n, const int* __restrict__ bh,
                                                    const int* __restrict__ base,
                                                    CurveRec* __restrict__ rec,
                                                    int* __restrict__ pos_of) {
   extern __shared__ int h[];
   const int nq = g.P * g.C + 1;
-  for (int q = threadIdx.x; q < nq; q += 256) h[q] = base[q] + bh[(size_t)blockIdx.x * nq + q];
+  for (int q = threadIdx.x; q < nq; q += kPartThreads) h[q] = base[q] + bh[(size_t)blockIdx.x * nq + q];
   __syncthreads();
   const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
-  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
+  for (int c = c0 + threadIdx.x; c < c1; c += kPartThreads) {
     const int pos = atomicAdd(&h[curve_bucket(g, c)], 1);
     const double* Pc = g.params + (size_t)c * 7;
     double sn, cs;
@@ -781,12 +782,13 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
   double2* rng = ws.get<double2>(irng);
   int* pos_of = ws.get<int>(ipos);
   const size_t hl = sizeof(int) * nq;
-  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(256), hl, pg, n, bh);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh);
   MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, kColScanQ)), dim3(kColScanQ * kColScanSeg),
              0, bh, nq, nblk, tot);
   MLC_LAUNCH(ctx, "lc_partition", k_part_base, dim3(1), dim3(1024), 0, tot, nq, P, pg.C, n, qbase, start,
              cnt);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(256), hl, pg, n, bh, qbase, rec, pos_of);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh, qbase, rec,
+             pos_of);
   if (a.nepoch > 0)
     MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
                a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);

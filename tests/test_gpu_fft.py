@@ -129,7 +129,8 @@ def test_unsupported_sizes_rejected(ctx):
                                                (256, 256, "uniform", 50.0),   # FP64 falls back to column FFTs
                                                (100, 120, "exponential", 3.1),
                                                (64, 1024, "gaussian", 6.0),   # compile-time row FFTs
-                                               (40, 2048, "uniform", 3.0)])
+                                               (40, 2048, "uniform", 3.0),
+                                               (24, 10240, "gaussian", 3.0)])  # FP64 node rows in two passes
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec):
     rng = np.random.default_rng(Ny + Nx)
@@ -176,3 +177,14 @@ def test_convolve_nodes_equals_one_by_one(ctx, Ny, Nx, prec):
     dst = (C.c_void_p * 1)(None)
     assert lib().mlc_map_convolve_nodes(ctx_h, 1, shp.ctypes.data, rad.ctypes.data, 1.0, None, None,
                                         C.addressof(dst)) != 0
+
+
+def test_context_close_closes_its_maps():
+    """Closing a context first closes the map spectra made on it (a map keeps
+    a pointer to its context), so a map that outlives its context is inert."""
+    c2 = api.Context(0)
+    spec = api.MapSpectrum(c2, torch.ones((16, 16), dtype=torch.float64, device="cuda"), "f32")
+    c2.close()
+    assert spec.h is None
+    spec.close()
+    del spec
