@@ -119,10 +119,8 @@ def run_step(ctx, ws, plain=False):
     chi2, grad = lcm.chi2_grad(ctx, ws["nodes"], ws["obs"], ws["params"], ws["prof"], ws["obs_idx"],
                                plain=plain)
     perm, _, sk = api.sort_f64(ctx, chi2, sorted_keys=True)
-    lo, npb = api.bin_edges(ctx, sk, ws["bins"])
-    sb = ws["sb"]
-    sb.lower_ind[:], sb.n_per_bin[:] = lo, npb
-    be = api.bin_chi2_GPU(ctx, sb, sk)
+    # bin table and binned means on the device: no host round trip in a step
+    lo, npb, _, be = api.bin_table_device(ctx, sk, ws["bins"])
     return chi2, grad, lo, npb, be
 
 
@@ -136,9 +134,8 @@ def setup(ctx, prec, m_dev, params, prof, times, mobs, sig):
     rr = [[[R0 * sc] for sc in BAND_SCALE] for _ in SHAPES]
     nodes = lcm.NodeSet(ctx, spec, SHAPES, rr, BAND_SCALE, compute=False)
     obs = lcm.Observations(ctx, len(BAND_SCALE), [[(times[b], mobs[b], sig[b]) for b in range(len(BAND_SCALE))]])
-    sb = api.Chi2SortBins(ctx, 1, NBINS)
     bins = api.mpd_bins(NBINS, 1.0, 1.0e7, True)
-    return dict(spec=spec, nodes=nodes, obs=obs, map=m_dev, sb=sb, bins=bins,
+    return dict(spec=spec, nodes=nodes, obs=obs, map=m_dev, bins=bins,
                 params=torch.tensor(params, device="cuda"), prof=torch.tensor(prof, device="cuda"),
                 obs_idx=torch.zeros(len(prof), dtype=torch.int32, device="cuda"))
 
@@ -289,6 +286,7 @@ def main():
         k64 = max(2, args.steps // 5)
         dt64, out64, _ = time_steps(ctx, ws64, k64, 1, plain=True, timing=False)
         chi64, grad64, lo64, npb64, be64 = out64
+        lo32, npb32, lo64, npb64 = (x.cpu().numpy() for x in (lo32, npb32, lo64, npb64))
         fin = torch.isfinite(chi64)
         rel = ((chi32 - chi64).abs() / chi64.abs())[fin]
         gs = grad64.abs().amax(dim=0)

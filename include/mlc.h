@@ -210,6 +210,18 @@ int mlc_bin_chi2(mlc_ctx* ctx, const double* d_chi2, int64_t n,
                  const uint32_t* lower_ind, const uint32_t* n_per_bin, int Nbins,
                  double* binned_chi2, double* binned_exp);
 
+/* The bin table and the binned means of n ascending keys without a host
+ * round trip: setup_integral_GPU's upper_bound + adjacent_difference
+ * (gpu_functions.cu:383-397) into d_lower / d_npb and bin_chi2_GPU
+ * (gpu_functions.cu:483-503) into d_binned_chi2 / d_binned_exp, all device
+ * arrays of Nbins; bins (host, Nbins right edges, non-decreasing, else
+ * MLC_ERR_ARG) is copied in.  The results
+ * equal mlc_bin_edges + mlc_bin_chi2 bit for bit; nothing is synchronised,
+ * so a caller's loop keeps the device busy. */
+int mlc_bin_table_device(mlc_ctx* ctx, const double* d_sorted, int64_t n, const double* bins,
+                         int Nbins, uint32_t* d_lower, uint32_t* d_npb, double* d_binned_chi2,
+                         double* d_binned_exp);
+
 /* ------------------------------------------------------------------ MPD
  * MagnificationMap::getBinnedMpd (magnification_map_gpu.cu:45-87): counts of
  * map values v with min <= v < max on right-edge bins (host `bins`, Nbins <= 4096),

@@ -58,6 +58,7 @@ SIGNATURES = {
     "mlc_setup_integral": (_i, [_p, _p, _p, _i, _i, _p, _i, _p, _p, _p]),
     "mlc_calculate_chi2": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
     "mlc_bin_chi2": (_i, [_p, _p, _i64, _p, _p, _i, _p, _p]),
+    "mlc_bin_table_device": (_i, [_p, _p, _i64, _p, _i, _p, _p, _p, _p]),
     "mlc_binned_mpd": (_i, [_p, _p, _i64, _i, _d, _d, _p, _p]),
     "mlc_chi2_multiband": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p]),
     # light curves

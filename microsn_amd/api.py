@@ -297,6 +297,22 @@ def bin_edges(ctx, sorted_keys, bins):
     return lo, npb
 
 
+def bin_table_device(ctx, sorted_keys, bins):
+    """(lower_ind, n_per_bin, binned_chi2, binned_exp) as device tensors
+    (uint32 stored as int32, FP64) for ascending keys; the same values as
+    bin_edges + bin_chi2_GPU, with no host round trip."""
+    _dev(ctx, sorted_keys, torch.float64, "keys")
+    bins = np.ascontiguousarray(bins, np.float64)
+    nb = bins.size
+    lo = torch.empty(nb, dtype=torch.int32, device=ctx.device)
+    npb = torch.empty(nb, dtype=torch.int32, device=ctx.device)
+    bc = torch.empty(nb, dtype=torch.float64, device=ctx.device)
+    be = torch.empty(nb, dtype=torch.float64, device=ctx.device)
+    ctx.check(lib().mlc_bin_table_device(ctx.h, ptr(sorted_keys), sorted_keys.numel(), ptr(bins), nb, ptr(lo),
+                                         ptr(npb), ptr(bc), ptr(be)))
+    return lo, npb, bc, be
+
+
 def binned_mpd(ctx, dmap, Nbins, vmin=0.02, vmax=200.0):
     """MagnificationMap::getBinnedMpd -> (bins, counts) host arrays."""
     bins = mpd_bins(Nbins, vmin, vmax, True)

@@ -23,6 +23,8 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
               double* sorted);
 void bin_edges(mlc_ctx* ctx, const double* sorted, int64_t n, const double* bins, int Nbins,
                unsigned* lower_ind, unsigned* n_per_bin);
+void bin_edges_device(mlc_ctx* ctx, const double* sorted, int64_t n, const double* d_bins, int Nbins,
+                      unsigned* d_upper, unsigned* d_lower, unsigned* d_npb);
 void map_convolve_value(mlc_map* m, const ProfileParams& P, void* d_value);
 ProfileParams checked_params(int shape, double rhalf, double pix, double rin, double a, int Nx, int Ny);
 }  // namespace mlc
@@ -112,6 +114,100 @@ __global__ __launch_bounds__(256) void k_bin_chunks(const double* __restrict__ c
   if (threadIdx.x == 0) {
     part[2 * c] = re[0];
     part[2 * c + 1] = rc[0];
+  }
+}
+
+// The chunk table of mlc_bin_chi2 built on the device (one workgroup): bin i
+// has ceil(npb[i] / kChunk) chunks, numbered consecutively in bin order;
+// cfirst[i] is its first chunk, cfirst[Nbins] the number of chunks.  With
+// non-decreasing edges there are at most maxc = Nbins + n / kChunk chunks;
+// no entry at or past maxc is written, and the count is clamped to maxc.
+__global__ __launch_bounds__(1024) void k_chunk_table(const unsigned* __restrict__ lower,
+                                                      const unsigned* __restrict__ npb, int nb,
+                                                      unsigned maxc,
+                                                      unsigned long long* __restrict__ cstart,
+                                                      unsigned* __restrict__ clen,
+                                                      unsigned* __restrict__ cfirst) {
+  __shared__ unsigned s[1024];
+  const int t = threadIdx.x, seg = (nb + 1023) / 1024;
+  const int a = min(nb, t * seg), e = min(nb, a + seg);
+  unsigned sum = 0;
+  for (int i = a; i < e; ++i) sum += (npb[i] + kChunk - 1) / kChunk;
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const unsigned v = t >= o ? s[t - o] : 0u;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  unsigned c = s[t] - sum;
+  for (int i = a; i < e; ++i) {
+    cfirst[i] = min(c, maxc);
+    unsigned long long st = lower[i];
+    for (unsigned left = npb[i]; left > 0 && c < maxc;) {
+      const unsigned len = min(left, (unsigned)kChunk);
+      cstart[c] = st;
+      clen[c] = len;
+      ++c;
+      st += len;
+      left -= len;
+    }
+  }
+  if (t == 1023) cfirst[nb] = min(s[1023], maxc);
+}
+
+// k_bin_chunks over a device chunk table of at most gridDim.x chunks.
+__global__ __launch_bounds__(256) void k_bin_chunks_dev(const double* __restrict__ chi2,
+                                                        const unsigned long long* __restrict__ cstart,
+                                                        const unsigned* __restrict__ clen,
+                                                        const unsigned* __restrict__ cfirst, int nb,
+                                                        double* __restrict__ part) {
+  if (blockIdx.x >= cfirst[nb]) return;  // uniform per workgroup
+  __shared__ double re[256], rc[256];
+  const int c = blockIdx.x;
+  const double* x = chi2 + cstart[c];
+  const unsigned len = clen[c];
+  double se = 0.0, sc = 0.0;
+  for (unsigned i = threadIdx.x; i < len; i += 256) {
+    double v = x[i];
+    se += exp(-0.5 * v);
+    sc += v;
+  }
+  re[threadIdx.x] = se;
+  rc[threadIdx.x] = sc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      re[threadIdx.x] += re[threadIdx.x + w];
+      rc[threadIdx.x] += rc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * c] = re[0];
+    part[2 * c + 1] = rc[0];
+  }
+}
+
+// Per-bin means from the chunk sums, added in chunk order as mlc_bin_chi2
+// does on the host; an empty bin gives the reference kernel's sentinel.
+__global__ void k_bin_finish(const double* __restrict__ part, const unsigned* __restrict__ cfirst,
+                             const unsigned* __restrict__ npb, int nb, double* __restrict__ bchi2,
+                             double* __restrict__ bexp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  double se = 0.0, sc = 0.0;
+  for (unsigned c = cfirst[i]; c < cfirst[i + 1]; ++c) {
+    se += part[2 * c];
+    sc += part[2 * c + 1];
+  }
+  if (npb[i] > 0) {
+    bexp[i] = se / npb[i];
+    bchi2[i] = sc / npb[i];
+  } else {  // kernel_bin_chi2 gpu_functions.cu:789-792
+    bexp[i] = exp(-2000.0 / 2.0);
+    bchi2[i] = 2000.0;
   }
 }
 
@@ -278,6 +374,37 @@ int mlc_bin_chi2(mlc_ctx* ctx, const double* d_chi2, int64_t n, const uint32_t* 
         binned_chi2[i] = 2000.0;
       }
     }
+  });
+}
+
+int mlc_bin_table_device(mlc_ctx* ctx, const double* d_sorted, int64_t n, const double* bins,
+                         int Nbins, uint32_t* d_lower, uint32_t* d_npb, double* d_binned_chi2,
+                         double* d_binned_exp) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(Nbins > 0 && bins && d_lower && d_npb && d_binned_chi2 && d_binned_exp,
+                "mlc_bin_table_device: bad bins");
+    MLC_REQUIRE(n >= 0 && n < ((int64_t)1 << 32) && (n == 0 || d_sorted), "mlc_bin_table_device: bad keys");
+    // non-decreasing edges keep every bin count >= 0 and their sum <= n
+    for (int i = 1; i < Nbins; ++i)
+      MLC_REQUIRE(bins[i] >= bins[i - 1], "mlc_bin_table_device: bin edges must be non-decreasing");
+    MLC_REQUIRE(bins[0] == bins[0], "mlc_bin_table_device: NaN bin edge");
+    // at most one partial chunk per bin plus the full ones
+    const int64_t maxc = Nbins + n / kChunk;
+    Workspace ws(ctx);
+    const int ib = ws.add(sizeof(double) * Nbins), iu = ws.add(sizeof(unsigned) * Nbins),
+              ics = ws.add(sizeof(unsigned long long) * maxc), icl = ws.add(sizeof(unsigned) * maxc),
+              icf = ws.add(sizeof(unsigned) * (Nbins + 1)), ipt = ws.add(sizeof(double) * 2 * maxc);
+    ws.commit();
+    double* db = ws.get<double>(ib);
+    h2d(ctx, db, bins, sizeof(double) * Nbins);
+    bin_edges_device(ctx, d_sorted, n, db, Nbins, ws.get<unsigned>(iu), d_lower, d_npb);
+    unsigned* cf = ws.get<unsigned>(icf);
+    MLC_LAUNCH(ctx, "bin_chunk_table", k_chunk_table, dim3(1), dim3(1024), 0, d_lower, d_npb, Nbins,
+               (unsigned)maxc, ws.get<unsigned long long>(ics), ws.get<unsigned>(icl), cf);
+    MLC_LAUNCH(ctx, "bin_chi2_chunks", k_bin_chunks_dev, dim3((unsigned)maxc), dim3(256), 0, d_sorted,
+               ws.get<unsigned long long>(ics), ws.get<unsigned>(icl), cf, Nbins, ws.get<double>(ipt));
+    MLC_LAUNCH(ctx, "bin_finish", k_bin_finish, dim3(cdiv(Nbins, 256)), dim3(256), 0, ws.get<double>(ipt),
+               cf, d_npb, Nbins, d_binned_chi2, d_binned_exp);
   });
 }
 

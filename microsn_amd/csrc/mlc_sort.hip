@@ -279,6 +279,17 @@ __global__ void k_upper_bound(const double* __restrict__ s, int64_t n, const dou
   upper[i] = (unsigned)lo;
 }
 
+// lower[i] = upper[i-1] (0 for i = 0), npb[i] = upper[i] - lower[i]
+// (setup_integral_GPU's adjacent_difference, gpu_functions.cu:389-397).
+__global__ void k_bin_table(const unsigned* __restrict__ upper, int nb, unsigned* __restrict__ lower,
+                            unsigned* __restrict__ npb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const unsigned lo = i ? upper[i - 1] : 0u;
+  lower[i] = lo;
+  npb[i] = upper[i] - lo;
+}
+
 template <int R>
 void sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* pbuf, unsigned* perm,
                  unsigned* cnt, const unsigned* hist, int64_t n) {
@@ -336,6 +347,16 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
     MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
                dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, keys,
                n, rank, sorted);
+}
+
+// The bin table on the device: d_lower / d_npb from the upper bounds, no
+// host round trip.
+void bin_edges_device(mlc_ctx* ctx, const double* sorted, int64_t n, const double* d_bins, int Nbins,
+                      unsigned* d_upper, unsigned* d_lower, unsigned* d_npb) {
+  MLC_LAUNCH(ctx, "bin_upper_bound", k_upper_bound, dim3(cdiv(Nbins, 64)), dim3(64), 0, sorted, n,
+             d_bins, Nbins, d_upper);
+  MLC_LAUNCH(ctx, "bin_table", k_bin_table, dim3(cdiv(Nbins, 256)), dim3(256), 0, d_upper, Nbins, d_lower,
+             d_npb);
 }
 
 void bin_edges(mlc_ctx* ctx, const double* sorted, int64_t n, const double* bins, int Nbins,

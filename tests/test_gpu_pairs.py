@@ -124,6 +124,37 @@ def test_bin_edges_match_upper_bound(ctx, orc):
     assert np.array_equal(lo, rlo) and np.array_equal(npb, rnpb)
 
 
+@pytest.mark.parametrize("n", [0, 7, 50_000, 300_000])
+def test_bin_table_device_equals_host_path(ctx, orc, n):
+    """mlc_bin_table_device against mlc_bin_edges + mlc_bin_chi2 bit for bit:
+    empty bins, an edge equal to keys, bins of more than one 4096-key chunk,
+    +inf and NaN keys past the last edge, and no keys at all."""
+    rng = np.random.default_rng(n + 5)
+    keys = np.sort(np.round(rng.lognormal(3, 1.5, n), 3))
+    if n:
+        keys[-3:] = np.inf
+        keys = np.concatenate([keys, [np.nan] * 2])
+    bins = orc.mpd_bins(60, 1e-2, 1e4, True)
+    if n > 1000:  # an edge equal to a key, the edges still ascending
+        k = int(np.searchsorted(bins, keys[1000]))
+        bins[k] = keys[1000]
+    assert np.all(np.diff(bins) >= 0)
+    d = _cuda(keys)
+    lo, npb, bc, be = api.bin_table_device(ctx, d, bins)
+    hlo, hnpb = api.bin_edges(ctx, d, bins)
+    assert np.array_equal(lo.cpu().numpy().view(np.uint32), hlo)
+    assert np.array_equal(npb.cpu().numpy().view(np.uint32), hnpb)
+    sb = api.Chi2SortBins(ctx, 1, bins.size)
+    sb.lower_ind[:], sb.n_per_bin[:] = hlo, hnpb
+    hbc, hbe = api.bin_chi2_GPU(ctx, sb, d)
+    assert np.array_equal(bc.cpu().numpy(), hbc) and np.array_equal(be.cpu().numpy(), hbe)
+    if n == 300_000:
+        assert hnpb.max() > 4096  # a bin of several 4096-key chunks
+    # descending edges are rejected before any launch
+    with pytest.raises(Exception, match="non-decreasing"):
+        api.bin_table_device(ctx, d, bins[::-1].copy())
+
+
 def _pair_case(orc, Nloc, Nprof, rng):
     LCA = 0.5 + rng.random((Nloc, Nprof)) * 4
     LCB = 0.5 + rng.random((Nloc, Nprof)) * 4
