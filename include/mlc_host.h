@@ -62,6 +62,14 @@ int mlc_gerlumph_write(const char* dir, double avgmu, double avgN, int Nx, doubl
  * Ny x Nx FP64 image as FLOAT_IMG, rows flipped, in the primary HDU and in an
  * image extension "NEW-EXTENSION". */
 int mlc_write_fits(const char* path, const double* data, int Nx, int Ny, int sampling);
+/* FITS image input, the optional map input (the reference reads maps only
+ * as GERLUMPH map.bin, magnification_map.cpp:7-40): HDU hdu (0 = primary)
+ * of a basic FITS file, 2-D, BITPIX 8/16/32/64/-32/-64, BSCALE/BZERO
+ * applied, rows flipped back to the orientation mlc_write_fits was given.
+ * Sets *Nx, *Ny; data (capacity cap) receives Ny x Nx FP64.  Returns 1 for
+ * bad arguments, 2 for a missing file, a malformed header or no such 2-D
+ * HDU, 3 when cap < Nx*Ny (the sizes are set). */
+int mlc_read_fits(const char* path, int hdu, double* data, int64_t cap, int* Nx, int* Ny);
 /* Image::writeImageBIN (image.cpp:4-18): the same samples as native FP64. */
 int mlc_write_bin(const char* path, const double* data, int Nx, int Ny, int sampling);
 

@@ -81,6 +81,7 @@ SIGNATURES = {
                                C.POINTER(C.c_double), C.POINTER(C.c_double), _p, _i64]),
     "mlc_gerlumph_write": (_i, [C.c_char_p, _d, _d, _i, _d, _d, _d, _d, _p]),
     "mlc_write_fits": (_i, [C.c_char_p, _p, _i, _i, _i]),
+    "mlc_read_fits": (_i, [C.c_char_p, _i, _p, _i64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mlc_write_bin": (_i, [C.c_char_p, _p, _i, _i, _i]),
     "mlc_size_parametric": (_d, [_d, _d, _d, _d]),
     "mlc_size_ss": (_d, [_d, _d, _d, _d]),

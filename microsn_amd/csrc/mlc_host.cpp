@@ -287,6 +287,119 @@ int mlc_write_fits(const char* path, const double* data, int Nx, int Ny, int sam
   return wr == out.size() ? 0 : 2;
 }
 
+namespace {
+// One header card's value: the text between "= " and a '/' comment, or the
+// quoted string.
+std::string fits_value(const char* card) {
+  std::string v(card + 10, 70);
+  const size_t q = v.find('\'');
+  if (q != std::string::npos && v.find_first_not_of(' ') == q) {
+    const size_t e = v.find('\'', q + 1);
+    return v.substr(q + 1, e == std::string::npos ? std::string::npos : e - q - 1);
+  }
+  const size_t sl = v.find('/');
+  if (sl != std::string::npos) v.resize(sl);
+  const size_t a = v.find_first_not_of(' '), b = v.find_last_not_of(' ');
+  return a == std::string::npos ? std::string() : v.substr(a, b - a + 1);
+}
+
+double fits_be(const unsigned char* p, int bitpix) {
+  uint64_t u = 0;
+  const int nb = std::abs(bitpix) / 8;
+  for (int i = 0; i < nb; ++i) u = (u << 8) | p[i];
+  switch (bitpix) {
+    case 8: return (double)(uint8_t)u;
+    case 16: return (double)(int16_t)(uint16_t)u;
+    case 32: return (double)(int32_t)(uint32_t)u;
+    case 64: return (double)(int64_t)u;
+    case -32: {
+      const uint32_t w = (uint32_t)u;
+      float f;
+      std::memcpy(&f, &w, 4);
+      return (double)f;
+    }
+    default: {
+      double d;
+      std::memcpy(&d, &u, 8);
+      return d;
+    }
+  }
+}
+}  // namespace
+
+// FITS image input (the optional map input; no reference reader): HDU `hdu`
+// (0 = primary) of a basic FITS file, 2-D, BITPIX 8/16/32/64/-32/-64 with
+// BSCALE/BZERO applied, returned with its rows flipped back, i.e. in the
+// orientation that mlc_write_fits (image.cpp:20-62) was given.
+int mlc_read_fits(const char* path, int hdu, double* data, int64_t cap, int* Nx, int* Ny) {
+  if (!path || hdu < 0 || !Nx || !Ny || cap < 0 || (cap > 0 && !data)) return 1;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return 2;
+  std::string raw;
+  char buf[1 << 16];
+  size_t got;
+  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) raw.append(buf, got);
+  std::fclose(f);
+  size_t pos = 0;
+  for (int h = 0;; ++h) {
+    int bitpix = 0, naxis = -1;
+    int64_t ax[3] = {0, 0, 0}, pcount = 0, gcount = 1;
+    double bscale = 1.0, bzero = 0.0;
+    bool end = false;
+    while (!end) {
+      if (pos + 2880 > raw.size()) return 2;
+      for (int c = 0; c < 36 && !end; ++c) {
+        const char* card = raw.data() + pos + 80 * c;
+        const std::string key = std::string(card, 8).substr(0, std::string(card, 8).find_last_not_of(' ') + 1);
+        if (key == "END") {
+          end = true;
+          break;
+        }
+        if (card[8] != '=' || card[9] != ' ') continue;
+        const std::string v = fits_value(card);
+        try {
+          if (key == "BITPIX") bitpix = std::stoi(v);
+          else if (key == "NAXIS") naxis = std::stoi(v);
+          else if (key == "NAXIS1") ax[0] = std::stoll(v);
+          else if (key == "NAXIS2") ax[1] = std::stoll(v);
+          else if (key == "NAXIS3") ax[2] = std::stoll(v);
+          else if (key == "PCOUNT") pcount = std::stoll(v);
+          else if (key == "GCOUNT") gcount = std::stoll(v);
+          else if (key == "BSCALE") bscale = std::stod(v);
+          else if (key == "BZERO") bzero = std::stod(v);
+        } catch (...) {
+          return 2;
+        }
+      }
+      pos += 2880;
+    }
+    if (bitpix != 8 && bitpix != 16 && bitpix != 32 && bitpix != 64 && bitpix != -32 && bitpix != -64)
+      return 2;
+    if (naxis < 0 || naxis > 3) return 2;
+    int64_t n = naxis == 0 ? 0 : 1;
+    for (int i = 0; i < naxis; ++i) {
+      if (ax[i] < 0) return 2;
+      n *= ax[i];
+    }
+    const int bpv = std::abs(bitpix) / 8;
+    const int64_t bytes = (int64_t)bpv * gcount * (pcount + n);
+    if (h == hdu) {
+      if (naxis != 2 || ax[0] < 1 || ax[1] < 1 || ax[0] > INT32_MAX || ax[1] > INT32_MAX) return 2;
+      if (pos + (size_t)n * bpv > raw.size()) return 2;
+      *Nx = (int)ax[0];
+      *Ny = (int)ax[1];
+      if (n > cap) return 3;
+      const unsigned char* p = reinterpret_cast<const unsigned char*>(raw.data() + pos);
+      for (int64_t i = 0; i < ax[1]; ++i)
+        for (int64_t j = 0; j < ax[0]; ++j)
+          data[(size_t)(ax[1] - 1 - i) * ax[0] + j] = bzero + bscale * fits_be(p + (size_t)(i * ax[0] + j) * bpv, bitpix);
+      return 0;
+    }
+    pos += (size_t)((bytes + 2879) / 2880 * 2880);
+    if (pos >= raw.size()) return 2;  // no such HDU
+  }
+}
+
 // Image::writeImageBIN (image.cpp:4-18): the sub-sampled image as native FP64.
 int mlc_write_bin(const char* path, const double* data, int Nx, int Ny, int sampling) {
   if (!path || !data || Nx <= 0 || Ny <= 0 || sampling <= 0 || Nx / sampling < 1 || Ny / sampling < 1)

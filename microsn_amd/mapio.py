@@ -1,7 +1,8 @@
 """Map input and output: GERLUMPH map directories (the reference's
 MagnificationMap constructor, simulating_lcurves/src/magnification_map.cpp:7-40)
 and the FITS / raw binary image writers of Image
-(simulating_lcurves/src/image.cpp:4-62).  Reading and writing run on the host
+(simulating_lcurves/src/image.cpp:4-62), and FITS images as an optional map
+input.  Reading and writing run on the host
 inside libmlc.so (mlc_host.h); the normalisation to magnifications runs on
 the device (mlc_gerlumph_normalise) or, for host arrays, here in FP64 with
 the same formula."""
@@ -83,6 +84,26 @@ def write_fits(path, image, sampling=1):
     a = np.ascontiguousarray(image, np.float64)
     if lib().mlc_write_fits(os.fsencode(path), ptr(a), a.shape[1], a.shape[0], int(sampling)):
         raise OSError(f"{path}: cannot write FITS")
+
+
+def read_fits_image(path, hdu=0):
+    """A 2-D FITS image HDU (0 = primary) as an (Ny, Nx) FP64 array in the
+    orientation write_fits was given (the optional FITS map input)."""
+    nx, ny = C.c_int(), C.c_int()
+    rc = lib().mlc_read_fits(os.fsencode(path), int(hdu), None, 0, C.byref(nx), C.byref(ny))
+    if rc not in (0, 3):
+        raise OSError(f"{path}: no readable 2-D image in HDU {hdu} (status {rc})")
+    a = np.empty((ny.value, nx.value))
+    if lib().mlc_read_fits(os.fsencode(path), int(hdu), ptr(a), a.size, C.byref(nx), C.byref(ny)):
+        raise OSError(f"{path}: cannot read HDU {hdu}")
+    return a
+
+
+def load_fits_map(ctx, path, hdu=0):
+    """A magnification map from a FITS image, as an FP64 device tensor."""
+    import torch
+
+    return torch.tensor(read_fits_image(path, hdu), device=ctx.device)
 
 
 def write_bin(path, image, sampling=1):

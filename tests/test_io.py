@@ -67,6 +67,60 @@ def test_fits_layout_flip_and_sampling(tmp_path):
     assert np.array_equal(a1, a0)
 
 
+def _fits_file(path, bitpix, arr, extra=(), naxis3=None):
+    """A one-HDU FITS file written by hand: fixed-format cards, big-endian
+    data, both padded to 2880 bytes."""
+    def card(k, v):
+        return (f"{k:<8}= {v:>20}").ljust(80).encode()
+    ny, nx = arr.shape
+    h = card("SIMPLE", "T") + card("BITPIX", str(bitpix))
+    h += card("NAXIS", "3" if naxis3 else "2") + card("NAXIS1", str(nx)) + card("NAXIS2", str(ny))
+    if naxis3:
+        h += card("NAXIS3", str(naxis3))
+    for k, v in extra:
+        h += card(k, v)
+    h += b"END".ljust(80)
+    h += b" " * (-len(h) % 2880)
+    dt = {16: ">i2", 32: ">i4", -32: ">f4", -64: ">f8", 8: ">u1"}[bitpix]
+    d = np.ascontiguousarray(arr).astype(dt).tobytes()
+    d += b"\0" * (-len(d) % 2880)
+    with open(path, "wb") as f:
+        f.write(h + d)
+
+
+def test_fits_reader_round_trip_and_formats(tmp_path):
+    """mlc_read_fits: both HDUs of the writer's files come back in the
+    writer's orientation; hand-made files with integer BITPIX and
+    BSCALE/BZERO, FP64 data, and malformed files."""
+    img = np.random.default_rng(2).random((12, 20)) * 50
+    p = str(tmp_path / "m.fits")
+    mapio.write_fits(p, img, sampling=2)
+    want = img[::2, ::2][:6, :10].astype(np.float32).astype(np.float64)
+    assert np.array_equal(mapio.read_fits_image(p, 0), want)
+    assert np.array_equal(mapio.read_fits_image(p, 1), want)
+    with pytest.raises(OSError):
+        mapio.read_fits_image(p, 2)
+    stored = np.arange(35, dtype=np.int16).reshape(5, 7) - 17
+    q = str(tmp_path / "i16.fits")
+    _fits_file(q, 16, stored, extra=(("BSCALE", "0.5"), ("BZERO", "100.0")))
+    assert np.array_equal(mapio.read_fits_image(q), (100.0 + 0.5 * stored.astype(np.float64))[::-1])
+    d64 = np.random.default_rng(3).standard_normal((4, 9))
+    q = str(tmp_path / "f64.fits")
+    _fits_file(q, -64, d64[::-1])
+    assert np.array_equal(mapio.read_fits_image(q), d64)
+    q = str(tmp_path / "cube.fits")
+    _fits_file(q, -32, np.zeros((4, 4)), naxis3=1)
+    with pytest.raises(OSError):
+        mapio.read_fits_image(q)
+    raw = open(p, "rb").read()
+    q = str(tmp_path / "short.fits")
+    open(q, "wb").write(raw[:2880 + 100])
+    with pytest.raises(OSError):
+        mapio.read_fits_image(q)
+    with pytest.raises(OSError):
+        mapio.read_fits_image(str(tmp_path / "missing.fits"))
+
+
 def test_bin_writer(tmp_path):
     img = np.random.default_rng(1).random((9, 12))
     p = str(tmp_path / "m.bin")
