@@ -34,14 +34,16 @@ def _pairs(path):
     return np.loadtxt(path)
 
 
-@pytest.mark.parametrize("shape", ["gaussian", "uniform"])
-def test_microlc_matches_oracle(tmp_path, orc, shape):
+@pytest.mark.parametrize("shape,loc", [("gaussian", "grid"), ("uniform", "grid"), ("gaussian", "random")])
+def test_microlc_matches_oracle(tmp_path, orc, shape, loc):
     assert os.path.exists(EXE), "microlc not built"
     N = 200
     dA, dB = _maps(tmp_path, N)
+    locations = ({"type": "grid", "Nmap": N, "ss": 5.0, "Nside": 12} if loc == "grid" else
+                 {"type": "random", "Nmap": N, "ss": 5.0, "seed": 321, "Nloc": 150})
     inp = {
         "sizes": [0, 0.05, 0.1, 0.2], "shape": shape,
-        "locations": {"type": "grid", "Nmap": N, "ss": 5.0, "Nside": 12},
+        "locations": locations,
         "mpds": {"mpd": {"Nbins": 60, "min": 0.01, "max": 50},
                  "ratio": {"Nbins": 40, "min": 0.01, "max": 30}},
         "DL": 1570, "DS": 1750, "DLS": 850,
@@ -52,7 +54,7 @@ def test_microlc_matches_oracle(tmp_path, orc, shape):
     p.write_text(json.dumps(inp))
     out = tmp_path / "out"
     out.mkdir()
-    print("microlc", shape, flush=True)
+    print("microlc", shape, loc, flush=True)
     r = subprocess.run([EXE, str(p)], cwd=out, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.strip().splitlines()
@@ -66,7 +68,10 @@ def test_microlc_matches_oracle(tmp_path, orc, shape):
     mA = orc.gerlumph_normalise(cA, metaA["avgmu"], metaA["avgN"], metaA["k"], metaA["g"])
     mB = orc.gerlumph_normalise(cB, metaB["avgmu"], metaB["avgN"], metaB["k"], metaB["g"])
     Noff = int(np.ceil(N / 5.0 * 0.2))
-    lx, ly = orc.locations_on_grid(12, N, Noff)
+    if loc == "grid":
+        lx, ly = orc.locations_on_grid(12, N, Noff)
+    else:  # rand_input.json's form: std::mt19937(seed), inclusive uniform ints
+        lx, ly = orc.locations_random(150, N, Noff, 321)
     LCA, DLCA = orc.expanding_source(mA, 5.0, inp["sizes"], shape, lx, ly)
     LCB, DLCB = orc.expanding_source(mB, 5.0, inp["sizes"], shape, lx, ly)
     assert np.allclose(_pairs(out / "lcurvesA.dat").reshape(LCA.shape), LCA, rtol=0, atol=6e-7)

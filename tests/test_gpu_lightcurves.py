@@ -104,10 +104,14 @@ def test_f32_matches_oracle_on_f32_maps(ctx, orc):
     prm[2, :4] = (Nx - 1 - 12.0, 50.0, 0.0, 1.0)       # x = Nx - 1 exactly at t = 12
     obsi[2] = 4
     obsi[3] = 3                                        # NaN epoch time
-    c, g = lcm.chi2_grad(ctx, nodes, obs, torch.tensor(prm, device="cuda"), torch.tensor(prof, device="cuda"),
-                         torch.tensor(obsi, device="cuda"))
+    bad_prof = prof.copy()
+    bad_prof[4], bad_prof[5] = -1, nodes.P              # no such profile: (+inf, 0)
+    c, g = lcm.chi2_grad(ctx, nodes, obs, torch.tensor(prm, device="cuda"),
+                         torch.tensor(bad_prof, device="cuda"), torch.tensor(obsi, device="cuda"))
     rc, rg = _oracle(orc, nodes, sets, prm, prof, obsi)
     c, g = c.cpu().numpy(), g.cpu().numpy()
+    assert np.all(np.isinf(c[4:6])) and not g[4:6].any()
+    rc[4:6], rg[4:6] = np.inf, 0.0
     fin = np.isfinite(rc)
     assert not fin[0] and not fin[3] and fin[2]
     assert np.array_equal(np.isfinite(c), fin)
