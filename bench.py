@@ -162,6 +162,41 @@ def time_steps(ctx, ws, steps, warmup, plain, timing):
     return dt, out, kern
 
 
+def capture_step(ctx, ws):
+    """One FP32 step captured into a HIP graph (torch.cuda.graph on a side
+    stream the library is pointed at); the step runs once on that stream
+    first so the workspace and device tables are in place.  Returns the
+    graph and the output tensors its replays write."""
+    import torch
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    ctx.set_stream(s)
+    try:
+        with torch.cuda.stream(s):
+            run_step(ctx, ws)
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            out = run_step(ctx, ws)
+    finally:
+        ctx.set_stream(torch.cuda.current_stream())
+    return g, out
+
+
+def time_graph(g, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
 def cpu_baseline(m, params, prof, times, mobs, sig, bins):
     """One whole step of the workload on the host with the FP64 oracle (C++,
     OpenMP): the 9 node maps (value and d/dr, each an FFT convolution of the
@@ -232,6 +267,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain leg")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the FP32 step as eager launches instead of replays of its captured graph")
     ap.add_argument("--pmc-step", action="store_true",
                     help="run two FP32 steps and a 512 MiB device copy (the byte-counter "
                          "calibration) for rocprofv3 --pmc, print nothing else")
@@ -271,7 +308,18 @@ def main():
     # ---- FP32 fused path (the metric)
     ws32 = setup(ctx, "f32", m_dev, params, prof, times, mobs, sig)
     t_setup = time.perf_counter() - t_setup
-    dt32, out32, _ = time_steps(ctx, ws32, args.steps, args.warmup, plain=False, timing=False)
+    dt_eager, out32, _ = time_steps(ctx, ws32, args.steps, args.warmup, plain=False, timing=False)
+    graph_info = None
+    dt32 = dt_eager
+    if not args.eager:
+        # the same step as one graph replay; its outputs must equal the eager step's
+        g32, gout = capture_step(ctx, ws32)
+        dt32 = time_graph(g32, args.steps, args.warmup)
+        same = all(torch.equal(a, b) for a, b in zip(gout, out32))
+        if not same:
+            print("graph replay differs from the eager step: timing the eager step", file=sys.stderr)
+            dt32 = dt_eager
+        graph_info = {"graph_replay_matches_eager": same, "ms_per_step_eager": dt_eager * 1e3}
     # the same steps again with a HIP event pair around every launch: the
     # per-kernel breakdown and the roofline's launch durations
     ksteps = max(1, min(args.steps, 5))
@@ -332,6 +380,7 @@ def main():
         "precision": err, "roofline": roof, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / ksteps for k, v in sorted(kern.items())},
         "ms_per_step_with_kernel_events": dt32_ev * 1e3,
+        "hip_graph": graph_info,
         "setup_s": t_setup,
     }
     print(json.dumps(line))

@@ -56,6 +56,14 @@ int mlc_ctx_create(int device, void* stream, mlc_ctx** out);
  * must be destroyed first: a map keeps a pointer to its context. */
 void mlc_ctx_destroy(mlc_ctx* ctx);
 const char* mlc_last_error(const mlc_ctx* ctx);
+/* All later work of the context goes to `stream`.  Graph capture: the calls
+ * of one light-curve step (mlc_map_update, mlc_map_convolve_nodes,
+ * mlc_lc_chi2_grad, mlc_sort_f64, mlc_bin_table_device) issue no host
+ * synchronisation, memset node or host copy once they have run with the same
+ * sizes and tables, so a second identical step can be captured on `stream`
+ * and replayed.  A captured graph refers to the context's workspace and
+ * device tables: a later call that needs a larger workspace, or uploads
+ * different bin edges or node jobs, invalidates it. */
 int mlc_ctx_set_stream(mlc_ctx* ctx, void* stream);
 int mlc_ctx_sync(mlc_ctx* ctx);
 int mlc_malloc(mlc_ctx* ctx, size_t bytes, void** d_ptr);

@@ -52,6 +52,12 @@ class Context:
         if rc != 0:
             raise MlcError(rc, lib().mlc_last_error(self.h).decode())
 
+    def set_stream(self, stream):
+        """Run the library's work on another torch stream (e.g. one that a
+        torch.cuda.graph capture records)."""
+        self.check(lib().mlc_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+        self.stream = stream
+
     def sync(self):
         self.check(lib().mlc_ctx_sync(self.h))
 

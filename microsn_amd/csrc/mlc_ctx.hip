@@ -35,6 +35,24 @@ int fill_grid(mlc_ctx* ctx, int per_cu) {
   return std::max(1, cus) * per_cu;
 }
 
+const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes) {
+  auto& t = ctx->tables[slot];
+  if (t.dev && t.host.size() == bytes && std::memcmp(t.host.data(), host, bytes) == 0) return t.dev;
+  if (bytes > t.cap) {
+    if (t.dev) {  // kernels queued earlier may still read it
+      hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(table)");
+      hip_check(hipFree(t.dev), "hipFree(table)");
+      t.dev = nullptr;
+      t.cap = 0;
+    }
+    hip_check(hipMalloc(&t.dev, std::max<size_t>(bytes, 256)), "hipMalloc(table)");
+    t.cap = std::max<size_t>(bytes, 256);
+  }
+  h2d(ctx, t.dev, host, bytes);
+  t.host.assign(static_cast<const char*>(host), static_cast<const char*>(host) + bytes);
+  return t.dev;
+}
+
 int resident_blocks(const void* kernel, int threads) {
   static std::mutex mu;
   static std::map<std::pair<const void*, int>, int> cache;
@@ -132,6 +150,7 @@ void mlc_ctx_destroy(mlc_ctx* ctx) {
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+  for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete ctx;

@@ -1039,14 +1039,13 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
     hmax = std::max(hmax, h);
   }
   Workspace ws(ctx);
-  int ij = ws.add(sizeof(NodeJob) * cnt);
   int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
   int ia = ws.add(sizeof(double) * std::max<int64_t>(na, 1)), iad = ws.add(sizeof(double) * std::max<int64_t>(na, 1));
   ws.commit();
-  NodeJob* dj = ws.get<NodeJob>(ij);
+  const NodeJob* dj =
+      static_cast<const NodeJob*>(const_table(ctx, kTableNodeJobs, jobs.data(), sizeof(NodeJob) * cnt));
   double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
   double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
-  h2d(ctx, dj, jobs.data(), sizeof(NodeJob) * cnt);
   MLC_LAUNCH(ctx, "profile", k_profile_many, dim3(cnt), dim3(1024), 0, dj, prof, dprof);
   MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many, dim3(cdiv(W, 256), hmax, cnt), dim3(256), 0, dj,
              prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad);

@@ -66,6 +66,14 @@ struct mlc_ctx {
 
   // twiddle tables: key = (n, precision) -> device array of n complex values
   std::map<std::pair<int, int>, void*> twiddles;
+
+  // small host tables kept on the device between calls (mlc::const_table)
+  struct ConstTable {
+    std::vector<char> host;
+    void* dev = nullptr;
+    size_t cap = 0;
+  };
+  std::map<int, ConstTable> tables;
 };
 
 namespace mlc {
@@ -167,6 +175,13 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 
 // Number of workgroups that fill the device a few times over (grid-stride kernels).
 int fill_grid(mlc_ctx* ctx, int per_cu);
+
+// A device copy of a small host table (bin edges, node jobs), one per slot:
+// uploaded when its bytes differ from the previous call's, reused otherwise,
+// so that a repeated call issues no host-to-device copy (and can be captured
+// into a graph once it has run).
+enum ConstSlot { kTableBins = 0, kTableNodeJobs = 1 };
+const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes);
 
 // Workgroups of `threads` lanes of `kernel` resident at once on one CU (the
 // occupancy limit of its registers and LDS), at least 1; cached per kernel.

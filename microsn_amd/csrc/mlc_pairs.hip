@@ -391,12 +391,11 @@ int mlc_bin_table_device(mlc_ctx* ctx, const double* d_sorted, int64_t n, const 
     // at most one partial chunk per bin plus the full ones
     const int64_t maxc = Nbins + n / kChunk;
     Workspace ws(ctx);
-    const int ib = ws.add(sizeof(double) * Nbins), iu = ws.add(sizeof(unsigned) * Nbins),
-              ics = ws.add(sizeof(unsigned long long) * maxc), icl = ws.add(sizeof(unsigned) * maxc),
-              icf = ws.add(sizeof(unsigned) * (Nbins + 1)), ipt = ws.add(sizeof(double) * 2 * maxc);
+    const int iu = ws.add(sizeof(unsigned) * Nbins), ics = ws.add(sizeof(unsigned long long) * maxc),
+              icl = ws.add(sizeof(unsigned) * maxc), icf = ws.add(sizeof(unsigned) * (Nbins + 1)),
+              ipt = ws.add(sizeof(double) * 2 * maxc);
     ws.commit();
-    double* db = ws.get<double>(ib);
-    h2d(ctx, db, bins, sizeof(double) * Nbins);
+    const double* db = static_cast<const double*>(const_table(ctx, kTableBins, bins, sizeof(double) * Nbins));
     bin_edges_device(ctx, d_sorted, n, db, Nbins, ws.get<unsigned>(iu), d_lower, d_npb);
     unsigned* cf = ws.get<unsigned>(icf);
     MLC_LAUNCH(ctx, "bin_chunk_table", k_chunk_table, dim3(1), dim3(1024), 0, d_lower, d_npb, Nbins,

@@ -50,6 +50,11 @@ __device__ __forceinline__ unsigned digit_of(uint64_t k, int pass) {
   return (unsigned)(k >> (8 * pass)) & 255u;
 }
 
+__global__ void k_zero_u32(unsigned* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 // Transform keys and histogram all eight digits.  hist: [8][256] uint32.
 __global__ __launch_bounds__(kThreads) void k_transform_hist(const double* __restrict__ in,
                                                              uint64_t* __restrict__ out, int64_t n,
@@ -337,7 +342,8 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   ws.commit();
   uint64_t *k0 = ws.get<uint64_t>(ik0), *k1 = ws.get<uint64_t>(ik1);
   unsigned *pbuf = ws.get<unsigned>(ip1), *hist = ws.get<unsigned>(ih), *cnt = ws.get<unsigned>(ic);
-  hip_check(hipMemsetAsync(hist, 0, sizeof(unsigned) * 8 * 256, ctx->stream), "hipMemsetAsync(hist)");
+  // a fill kernel, not a memset node, so that a captured graph replays it
+  MLC_LAUNCH(ctx, "sort_zero_hist", k_zero_u32, dim3(8), dim3(256), 0, hist, 8 * 256);
   const int g = std::min<int64_t>(ntiles, fill_grid(ctx, 1));
   MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
              hist);
