@@ -77,16 +77,35 @@ def test_uniform_disc_pixel_count():
         assert np.allclose(p[inside], 1.0 / inside.sum(), rtol=1e-15)
 
 
-@pytest.mark.parametrize("shape", ["gaussian", "exponential", "gaussian_hole"])
-def test_profile_derivative_matches_central_difference(shape):
+@pytest.mark.parametrize("shape,rin,a", [("gaussian", 0.0, 1.0), ("exponential", 0.0, 1.0),
+                                         ("gaussian_hole", 1.0, 1.0), ("thermal_hole", 1.0, 1.0),
+                                         ("wavy", 0.0, 1.0), ("wavy", 0.0, -0.6)])
+def test_profile_derivative_matches_central_difference(shape, rin, a):
     rhalf, h = 3.3, 1e-5
-    p, dp = orc.profile(shape, rhalf, 1.0, deriv=True)
+    p, dp = orc.profile(shape, rhalf, 1.0, rin=rin, a=a, deriv=True)
     n = p.shape[0]
-    pp = orc.profile(shape, rhalf + h, 1.0)
-    pm = orc.profile(shape, rhalf - h, 1.0)
+    pp = orc.profile(shape, rhalf + h, 1.0, rin=rin, a=a)
+    pm = orc.profile(shape, rhalf - h, 1.0, rin=rin, a=a)
     assert pp.shape[0] == n == pm.shape[0]
     fd = (pp - pm) / (2 * h)
     assert np.max(np.abs(fd - dp)) < 1e-6 * np.max(np.abs(dp))
+
+
+def test_thermal_hole_and_wavy_invariants():
+    """Properties the reference's definitions fix (profile.cpp:315-398)
+    without restating their formulas: the thermal hole is dark inside
+    Rin = rhalf / 4 and bright outside; the wavy profile is dark from
+    2 rhalf outward, and its amplitude a drops out of the normalised profile
+    (sign included)."""
+    for rhalf in (4.0, 8.0, 13.5):
+        p = orc.profile("thermal_hole", rhalf, 1.0)
+        r = _centres(p.shape[0], 1.0)
+        assert np.all(p[r <= rhalf / 4] == 0.0) and np.all(p[r > rhalf / 4] > 0.0)
+        w1 = orc.profile("wavy", rhalf, 1.0, a=1.0)
+        r = _centres(w1.shape[0], 1.0)
+        assert np.all(w1[r >= 2 * rhalf] == 0.0) and np.all(w1[r < 2 * rhalf] >= 0.0)
+        for a in (0.3, -0.6, 7.0):
+            assert np.allclose(orc.profile("wavy", rhalf, 1.0, a=a), w1, rtol=1e-13, atol=0)
 
 
 # ---------------------------------------------------------------- FFT and convolution
