@@ -13,9 +13,13 @@
 // kernels reduce to a copy.  The decision is taken on the device by every
 // kernel of the pass, so the sort never waits for the host.
 //
+// Up to 2^21 keys: segments of 2048 keys are sorted in LDS (all eight passes
+// without leaving the workgroup), then merged pairwise (stable merge path,
+// ceil(log2(n/2048)) rounds).  Larger sorts run eight passes over the whole
+// array.
+//
 // One pass over tiles of 256*R keys (256 threads x R rounds, round r holds
-// keys r*256 + t, i.e. tile order = (round, thread); R = 4 for small sorts,
-// 16 for large ones):
+// keys r*256 + t, i.e. tile order = (round, thread); R = 16):
 //   k_tile_hist  per-tile digit counts, digit-major [256][ntiles]
 //   k_scan_tiles exclusive offsets: digit base + earlier tiles of that digit
 //   k_scatter    stable rank inside the tile (wavefront match on the 8 digit
@@ -32,7 +36,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int64_t kSmallSort = 1 << 21;  // up to this many keys: 1024-key tiles
+constexpr int64_t kSmallSort = 1 << 21;  // up to this many keys: segment sorts and merges
 
 // True when every key of the pass has the same digit (block-uniform).
 __device__ __forceinline__ bool pass_trivial(const unsigned* __restrict__ hist_pass, int64_t n) {
@@ -249,6 +253,183 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
   }
 }
 
+// ---- sorts of up to kSmallSort keys: segments sorted in LDS, then merged.
+constexpr int kSegThreads = 256, kSegR = 8, kSeg = kSegThreads * kSegR;  // keys per segment
+constexpr int kMergeThreads = 256, kMergeItems = 8;  // outputs per thread of a merge round
+constexpr int kMergeTile = kMergeThreads * kMergeItems;  // outputs per workgroup
+static_assert((2 * kSeg) % kMergeTile == 0, "a merge workgroup stays inside one pair of runs");
+
+// Stable sort of one segment of kSeg transformed keys, with their original
+// indices as payload, entirely in LDS: eight 8-bit LSD passes, each ranking
+// keys stably as k_scatter does (segment order = (round, thread)); passes
+// whose digit is the same for every key of the whole input are skipped.
+__global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __restrict__ kin, int64_t n,
+                                                          const unsigned* __restrict__ hist,
+                                                          uint64_t* __restrict__ kout,
+                                                          unsigned* __restrict__ pout) {
+  __shared__ uint64_t sk[kSeg];
+  __shared__ unsigned sp[kSeg];
+  __shared__ unsigned wave_cnt[kWaves][256];
+  __shared__ unsigned run[256];    // keys of each digit ranked so far in this pass
+  __shared__ unsigned start[256];  // first slot of each digit (exclusive scan)
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kSeg;
+  const int valid = (int)min<int64_t>(kSeg, n - base);
+  for (int q = t; q < kSeg; q += kSegThreads) {
+    sk[q] = q < valid ? kin[base + q] : ~0ull;
+    sp[q] = (unsigned)(base + q);
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int pass = 0; pass < 8; ++pass) {
+    if (pass_trivial(hist + pass * 256, n)) continue;  // block-uniform; also a barrier
+    run[t] = 0;
+    for (int i = t; i < kWaves * 256; i += kSegThreads) (&wave_cnt[0][0])[i] = 0;
+    uint64_t key[kSegR];
+    unsigned pay[kSegR], rank[kSegR], dig[kSegR];
+#pragma unroll
+    for (int r = 0; r < kSegR; ++r) {
+      const int q = r * kSegThreads + t;
+      key[r] = sk[q];
+      pay[r] = sp[q];
+      dig[r] = q < valid ? digit_of(key[r], pass) : 256u;  // 256: padding, not ranked
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSegR; ++r) {
+      const unsigned d = dig[r];
+      // lanes of this wave holding the same digit (padding lanes match nobody)
+      uint64_t peers = __ballot(d < 256u);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        uint64_t m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+      }
+      if (d >= 256u) peers = 0;
+      const unsigned before = (unsigned)__popcll(peers & lt);
+      if (d < 256u && before == 0) wave_cnt[w][d] = (unsigned)__popcll(peers);
+      __syncthreads();
+      if (d < 256u) {
+        unsigned p = run[d] + before;
+        for (int ww = 0; ww < w; ++ww) p += wave_cnt[ww][d];
+        rank[r] = p;
+      }
+      __syncthreads();
+      {
+        unsigned add = 0;
+        for (int ww = 0; ww < kWaves; ++ww) {
+          add += wave_cnt[ww][t];
+          wave_cnt[ww][t] = 0;
+        }
+        run[t] += add;
+      }
+      __syncthreads();
+    }
+    {  // exclusive scan of the digit counts (run[]) -> start[]
+      const unsigned v = run[t];
+      start[t] = v;
+      __syncthreads();
+      for (int off = 1; off < 256; off <<= 1) {
+        const unsigned a = t >= off ? start[t - off] : 0u;
+        __syncthreads();
+        start[t] += a;
+        __syncthreads();
+      }
+      const unsigned incl = start[t];
+      __syncthreads();
+      start[t] = incl - v;
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < kSegR; ++r) {
+      if (dig[r] < 256u) {
+        const unsigned q = start[dig[r]] + rank[r];
+        sk[q] = key[r];
+        sp[q] = pay[r];
+      }
+    }
+    __syncthreads();
+  }
+  for (int q = t; q < valid; q += kSegThreads) {
+    kout[base + q] = sk[q];
+    pout[base + q] = sp[q];
+  }
+}
+
+// Merge path: the number of elements of the left run A (length la) among
+// the first d outputs of the stable merge of A and B (length lb), i.e. the
+// first i with A[i] > B[d - 1 - i].  One wavefront searches together: 64
+// probes per step narrow the range 64-fold, so a run of 2^20 needs four
+// dependent rounds of reads instead of twenty.
+__device__ __forceinline__ int64_t merge_split_wave(const uint64_t* __restrict__ A, int64_t la,
+                                                    const uint64_t* __restrict__ B, int64_t lb, int64_t d,
+                                                    int lane) {
+  int64_t lo = max<int64_t>(0, d - lb), hi = min<int64_t>(d, la);
+  while (hi > lo) {  // uniform per wave
+    const int64_t span = hi - lo;
+    const int64_t p = span <= 64 ? lo + lane : lo + span * lane / 64;
+    const bool in = p < hi && A[p] <= B[d - 1 - p];
+    const int cnt = __popcll(__ballot(in));  // the probes that hold form a prefix
+    if (span <= 64) return lo + cnt;
+    const int64_t p_last = lo + span * (cnt - 1) / 64, p_first_out = lo + span * cnt / 64;
+    const int64_t nlo = cnt == 0 ? lo : p_last + 1;
+    const int64_t nhi = cnt == 64 ? hi : p_first_out;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo;
+}
+
+// One merge round: adjacent sorted runs of length L (the last ones may be
+// shorter) become runs of 2L.  A workgroup writes kMergeTile consecutive
+// outputs, all inside one pair of runs (2L is a multiple of kMergeTile):
+// two wavefronts find where its first and last outputs split the pair, the
+// workgroup copies those stretches of both runs to LDS, and each thread
+// merges kMergeItems outputs there after its own merge-path search.
+// Stable: on equal keys the left run's element comes first.
+__global__ __launch_bounds__(kMergeThreads) void k_merge(const uint64_t* __restrict__ kin,
+                                                         const unsigned* __restrict__ pin,
+                                                         uint64_t* __restrict__ kout,
+                                                         unsigned* __restrict__ pout, int64_t n, int64_t L) {
+  __shared__ uint64_t sk[kMergeTile];
+  __shared__ unsigned sp[kMergeTile];
+  __shared__ int64_t split[2];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t o0 = (int64_t)blockIdx.x * kMergeTile;
+  const int64_t q0 = o0 / (2 * L) * (2 * L);
+  const int64_t a0 = q0, a1 = min(q0 + L, n), b0 = a1, b1 = min(q0 + 2 * L, n);
+  const int64_t la = a1 - a0, lb = b1 - b0;
+  const int64_t d0 = o0 - q0, d1 = min<int64_t>(d0 + kMergeTile, la + lb);
+  if (w < 2) {
+    const int64_t sp_ = merge_split_wave(kin + a0, la, kin + b0, lb, w == 0 ? d0 : d1, lane);
+    if (lane == 0) split[w] = sp_;
+  }
+  __syncthreads();
+  const int64_t i0 = split[0], i1 = split[1];
+  const int na = (int)(i1 - i0), nb = (int)((d1 - i1) - (d0 - i0));
+  for (int q = t; q < na + nb; q += kMergeThreads) {
+    const int64_t src = q < na ? a0 + i0 + q : b0 + (d0 - i0) + (q - na);
+    sk[q] = kin[src];
+    sp[q] = pin[src];
+  }
+  __syncthreads();
+  const int e0 = t * kMergeItems, ne = (int)(d1 - d0);
+  if (e0 >= ne) return;
+  int lo = max(0, e0 - nb), hi = min(e0, na);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sk[mid] <= sk[na + e0 - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo, j = na + (e0 - lo);
+  const int jend = na + nb;
+  for (int e = 0; e < kMergeItems && e0 + e < ne; ++e) {
+    const bool take_a = j >= jend || (i < na && sk[i] <= sk[j]);
+    const int src = take_a ? i++ : j++;
+    kout[o0 + e0 + e] = sk[src];
+    pout[o0 + e0 + e] = sp[src];
+  }
+}
+
 __global__ void k_rank_and_gather(const unsigned* __restrict__ perm, const double* __restrict__ keys,
                                   int64_t n, unsigned* __restrict__ rank, double* __restrict__ sorted) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -332,8 +513,8 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   if (n <= 0) return;
   MLC_REQUIRE(n < ((int64_t)1 << 32), "sort: more than 2^32-1 keys");
   const bool small = n <= kSmallSort;
-  const int64_t tile = kThreads * (small ? 4 : 16);
-  const int64_t ntiles = (n + tile - 1) / tile;
+  const int64_t tile = kThreads * 16;
+  const int64_t ntiles = small ? 1 : (n + tile - 1) / tile;
   Workspace ws(ctx);
   int ik0 = ws.add(sizeof(uint64_t) * n), ik1 = ws.add(sizeof(uint64_t) * n);
   int ip1 = ws.add(sizeof(unsigned) * n);
@@ -344,11 +525,28 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   unsigned *pbuf = ws.get<unsigned>(ip1), *hist = ws.get<unsigned>(ih), *cnt = ws.get<unsigned>(ic);
   // a fill kernel, not a memset node, so that a captured graph replays it
   MLC_LAUNCH(ctx, "sort_zero_hist", k_zero_u32, dim3(8), dim3(256), 0, hist, 8 * 256);
-  const int g = std::min<int64_t>(ntiles, fill_grid(ctx, 1));
+  const int g = (int)std::min<int64_t>(cdiv(n, kThreads * 4), fill_grid(ctx, 1));
   MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
              hist);
-  if (small) sort_passes<4>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
-  else sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
+  if (small) {
+    // segments, then merge rounds; the segment sort writes the payload where
+    // an even number of rounds leaves it in perm
+    int rounds = 0;
+    for (int64_t L = kSeg; L < n; L *= 2) ++rounds;
+    unsigned* p0 = rounds % 2 == 0 ? perm : pbuf;
+    MLC_LAUNCH(ctx, "sort_segments", k_seg_sort, dim3(cdiv(n, kSeg)), dim3(kSegThreads), 0, k0, n, hist, k1,
+               p0);
+    uint64_t *kin = k1, *kout = k0;
+    unsigned *pin = p0, *pout = p0 == perm ? pbuf : perm;
+    for (int64_t L = kSeg; L < n; L *= 2) {
+      MLC_LAUNCH(ctx, "sort_merge", k_merge, dim3(cdiv(n, kMergeTile)), dim3(kMergeThreads), 0, kin, pin,
+                 kout, pout, n, L);
+      std::swap(kin, kout);
+      std::swap(pin, pout);
+    }
+  } else {
+    sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
+  }
   if (rank || sorted)
     MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
                dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, keys,
