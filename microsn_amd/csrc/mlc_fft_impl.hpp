@@ -112,7 +112,6 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
                                                const cpx<T>* __restrict__ twM,
                                                const cpx<T>* __restrict__ twN) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[G / 64];
   const int y = blockIdx.x, g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
   cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* buf = q + QL;
@@ -129,6 +128,9 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
     if (n < M) v[i] = row[n];
   }
   if (rowsum) {
+    // the row buffer, not yet filled, holds the per-wave partial sums (no
+    // static LDS: the widest rows use all of it)
+    double* red = reinterpret_cast<double*>(buf);
     double a = 0.0;
 #pragma unroll
     for (int i = 0; i < kPer; ++i)
@@ -139,6 +141,7 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
     double tot = 0.0;
 #pragma unroll
     for (int w = 0; w < G / 64; ++w) tot += red[w];
+    __syncthreads();  // every wave has read the sums before buf is filled
     if (g == 0) rowsum[y] = tot;
     sh = tot / (double)Nx;
   }
@@ -542,6 +545,56 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
   }
 }
 
+// The map's row spectrum with the compile-time FFT of M = Nx/2 points (one
+// row per workgroup): the row minus its own FP64 mean, packed as z[n] =
+// x[2n] + i x[2n+1], one forward DFT_M, then the split into the half
+// spectrum X[0..M] (r2c_post).  The FP64 row sum goes to rowsum[y];
+// k_row_dc_fix sets the DC terms afterwards.
+template <class T, int M>
+__global__ __launch_bounds__(CtGeom<M>::G) void k_map_rows(const double* __restrict__ in,
+                                                         cpx<T>* __restrict__ out,
+                                                         double* __restrict__ rowsum,
+                                                         const cpx<T>* __restrict__ tw,
+                                                         const cpx<T>* __restrict__ twN) {
+  using Gm = CtGeom<M>;
+  constexpr int G = Gm::G, W = M + 1, NU = M / G;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[(G + 63) / 64];
+  cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* hi = lo + Gm::NLO;
+  cpx<T>* buf = hi + Gm::NHI;
+  const int y = blockIdx.x, t = threadIdx.x;
+  for (int i = t; i < Gm::NLO + Gm::NHI; i += G) lo[i] = tw[i];
+  const cpx<double>* row = reinterpret_cast<const cpx<double>*>(in + (size_t)y * 2 * M);
+  cpx<double> v[NU];
+#pragma unroll
+  for (int i = 0; i < NU; ++i) v[i] = row[t + i * G];
+  double a = 0.0;
+#pragma unroll
+  for (int i = 0; i < NU; ++i) a += v[i].x + v[i].y;
+  // workgroups of fewer than 64 threads reduce over their own lanes only
+  for (int o = (G < 64 ? G : 64) / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  if ((t & 63) == 0) red[t >> 6] = a;
+  __syncthreads();
+  double tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < (G + 63) / 64; ++w) tot += red[w];
+  if (t == 0) rowsum[y] = tot;
+  const double mean = tot / (2.0 * M);
+#pragma unroll
+  for (int i = 0; i < NU; ++i)
+    buf[pad16(t + i * G)] = mk<T>((T)(v[i].x - mean), (T)(v[i].y - mean));
+  __syncthreads();
+  ct_fft<T, M, false>(buf, lo, hi, t);
+  cpx<T>* o = out + (size_t)y * W;
+  for (int k = t; k <= M / 2; k += G) {
+    cpx<T> Xk, Xm;
+    r2c_post(buf[pad16(k)], buf[pad16(k == 0 ? 0 : M - k)], twN[k], &Xk, &Xm);
+    o[k] = Xk;
+    if (M - k != k) o[M - k] = Xm;
+  }
+}
+
 // ------------------------------------------------------------------ profiles
 // One workgroup evaluates the n x n profile and its d/drhalf, normalises both
 // (K = f/S, dK = (df - K dS)/S) in FP64.
@@ -883,20 +936,23 @@ void node_col(mlc_ctx* ctx, const cpx<T>* fmap, const double* av, const double* 
   });
 }
 
-// Inverse twiddle tables of mlc_ctfft.hpp for length N: lo[l] = w^l,
-// l < 2^LB, then hi[h] = w^(h 2^LB), w = exp(2 pi i/N); cached per context.
-template <class T, int N>
+// Twiddle tables of mlc_ctfft.hpp for length N: lo[l] = w^l, l < 2^LB,
+// then hi[h] = w^(h 2^LB), with w = exp(+2 pi i/N) for the inverse transform
+// and exp(-2 pi i/N) for the forward one (the stages use the table as
+// given); cached per context and direction.
+template <class T, int N, bool INV>
 const cpx<T>* ct_twiddles(mlc_ctx* ctx) {
   using Gm = CtGeom<N>;
-  auto key = std::make_pair(N, 3000 + (int)sizeof(T));
+  auto key = std::make_pair(N, (INV ? 3000 : 4000) + (int)sizeof(T));
   auto it = ctx->twiddles.find(key);
   if (it != ctx->twiddles.end()) return (const cpx<T>*)it->second;
   const long double pi = 3.14159265358979323846264338327950288L;
+  const long double sg = INV ? 1.0L : -1.0L;
   std::vector<cpx<T>> h;
   for (int l = 0; l < Gm::NLO; ++l)
-    h.push_back(mk<T>((T)std::cos(2 * pi * l / N), (T)std::sin(2 * pi * l / N)));
+    h.push_back(mk<T>((T)std::cos(sg * 2 * pi * l / N), (T)std::sin(sg * 2 * pi * l / N)));
   for (int k = 0; k < Gm::NHI; ++k) {
-    const long double a = 2 * pi * (long double)k * Gm::NLO / N;
+    const long double a = sg * 2 * pi * (long double)k * Gm::NLO / N;
     h.push_back(mk<T>((T)std::cos(a), (T)std::sin(a)));
   }
   void* d = nullptr;
@@ -910,6 +966,35 @@ const cpx<T>* ct_twiddles(mlc_ctx* ctx) {
 // Node maps from the node spectra with the compile-time row FFT; false for
 // widths it does not cover (not a power of two in [256, 16384], or rows
 // that do not fit in LDS).
+// k_map_rows for Nx = 2M with M a power of two from 256 to 8192; false for
+// other widths (the runtime-plan row kernel takes them).
+template <class T>
+bool map_rows(mlc_ctx* ctx, const double* in, cpx<T>* out, double* rowsum, int Nx, int Ny) {
+  bool done = false;
+  auto go = [&](auto mc) {
+    constexpr int M = decltype(mc)::value;
+    using Gm = CtGeom<M>;
+    const size_t lds = sizeof(cpx<T>) * (size_t)(Gm::NLO + Gm::NHI + Gm::LDS);
+    if (lds > kLdsBytes - 1024) return;
+    auto k = k_map_rows<T, M>;
+    allow_lds(k, lds);
+    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(Gm::G), lds, in, out, rowsum, ct_twiddles<T, M, false>(ctx),
+               twiddles<T>(ctx, Nx));
+    done = true;
+  };
+  using std::integral_constant;
+  switch (Nx / 2) {
+    case 256: go(integral_constant<int, 256>{}); break;
+    case 512: go(integral_constant<int, 512>{}); break;
+    case 1024: go(integral_constant<int, 1024>{}); break;
+    case 2048: go(integral_constant<int, 2048>{}); break;
+    case 4096: go(integral_constant<int, 4096>{}); break;
+    case 8192: go(integral_constant<int, 8192>{}); break;
+    default: break;
+  }
+  return done;
+}
+
 template <class T>
 bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx, int Ny,
                const double* shift) {
@@ -922,7 +1007,7 @@ bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx,
     auto k = k_node_rows<T, N>;
     allow_lds(k, lds);
     MLC_LAUNCH(ctx, "fft_row_c2r_node", k, dim3(((Ny + 7) / 8) * 8), dim3(Gm::G), lds, Hv, Hd,
-               (cpx<T>*)out, Ny, shift, ct_twiddles<T, N>(ctx));
+               (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
     done = true;
   };
   using std::integral_constant;
@@ -970,7 +1055,8 @@ void fft_map_transform(mlc_map* m, const double* d_map) {
     int ir = ws.add(sizeof(double) * m->Ny);
     ws.commit();
     double* rowsum = ws.get<double>(ir);
-    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, rowsum);
+    if (!map_rows<T>(ctx, d_map, (cpx<T>*)m->rows, rowsum, m->Nx, m->Ny))
+      row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, rowsum);
     MLC_LAUNCH(ctx, "map_dc", k_row_dc_fix<T>, dim3(1), dim3(1024), 0, rowsum, m->Nx, m->Ny,
                (cpx<T>*)m->rows, m->mean);
   } else {

@@ -130,7 +130,8 @@ def test_unsupported_sizes_rejected(ctx):
                                                (100, 120, "exponential", 3.1),
                                                (64, 1024, "gaussian", 6.0),   # compile-time row FFTs
                                                (40, 2048, "uniform", 3.0),
-                                               (24, 10240, "gaussian", 3.0)])  # FP64 node rows in two passes
+                                               (24, 10240, "gaussian", 3.0),   # FP64 node rows in two passes
+                                               (16, 16384, "uniform", 2.0)])   # longest compile-time rows
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec):
     rng = np.random.default_rng(Ny + Nx)
