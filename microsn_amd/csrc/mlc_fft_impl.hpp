@@ -545,6 +545,124 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
   }
 }
 
+// Node-map rows straight from the map's row spectrum: the y-filter of
+// k_node_yfilter and the row transform of k_node_rows in one kernel, so the
+// two node spectra never leave the chip.  A workgroup of R transform groups
+// (R CtGeom<N>::G threads) owns R consecutive rows y0 .. y0+R-1.  First every
+// thread runs the y-filter for its 8/R columns u < N/2 and the R rows, with
+// the inputs G[y0+i-k] and G[y0+i+k+1] of a block of kNfK taps in two
+// register windows that slide by kNfK rows per block (the next block's rows
+// and taps are loaded before the current block's arithmetic), and puts
+// Z[u] = Hv + i Hd and Z[N-u] into the R row buffers in LDS.  Then group g
+// transforms row y0+g and writes it into the tiled node map.  Taps tp are
+// the (value, derivative) pairs in the working precision, zero beyond h; the
+// sums, their order and the final products are those of k_node_yfilter and
+// k_node_rows.  Workgroups run XCD by XCD over consecutive row blocks, so
+// the halo rows of G are shared in one L2.
+constexpr int kNfK = 4;
+
+template <class T, int N, int R>
+__global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
+    const cpx<T>* __restrict__ Gs, const cpx<T>* __restrict__ tp, int h, int hp,
+    const double* __restrict__ phx, cpx<T>* __restrict__ out, int Ny, const double* __restrict__ shift,
+    const cpx<T>* __restrict__ tw) {
+  using Gm = CtGeom<N>;
+  constexpr int G = Gm::G, NT = G * R, W = N / 2 + 1, CPT = (N / 2) / NT, NW = R + kNfK - 1;
+  static_assert((N / 2) % NT == 0, "columns split evenly over the threads");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* hi = lo + Gm::NLO;
+  cpx<T>* bufs = hi + Gm::NHI;  // R buffers of Gm::LDS elements
+  const int nq = (Ny + R - 1) / R;
+  const int q = xcd_block(nq);
+  if (q >= nq) return;
+  const int y0 = q * R, t = threadIdx.x;
+  for (int i = t; i < Gm::NLO + Gm::NHI; i += NT) lo[i] = tw[i];
+  const double sc = 0.5 / (double)N;
+  // row r of the spectrum, r in [-Ny, 2 Ny)
+  auto grow = [&](int r, int u) -> cpx<T> {
+    r = r < 0 ? r + Ny : (r >= Ny ? r - Ny : r);
+    return Gs[(size_t)r * W + u];
+  };
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int u = t + c * NT;
+    const cpx<T> ph = mk<T>((T)(phx[2 * u] * sc), (T)(phx[2 * u + 1] * sc));
+    cpx<T> accv[R], accd[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) accv[i] = accd[i] = mk<T>(0, 0);
+    // L[j] = G[y0 - kb - (kNfK-1) + j], Rw[j] = G[y0 + kb + 1 + j], j < NW
+    cpx<T> L[NW], Rw[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      L[j] = grow(y0 - (kNfK - 1) + j, u);
+      Rw[j] = grow(y0 + 1 + j, u);
+    }
+    cpx<T> tk[kNfK];
+#pragma unroll
+    for (int kk = 0; kk < kNfK; ++kk) tk[kk] = kk < h ? tp[(size_t)kk * W + u] : mk<T>(0, 0);
+    for (int kb = 0; kb < hp; kb += kNfK) {
+      // the next block's new rows and taps, in flight during this block's sums
+      const bool more = kb + kNfK < hp;
+      cpx<T> nL[kNfK], nR[kNfK], nt[kNfK];
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) {
+          nL[j] = grow(y0 - kb - kNfK - (kNfK - 1) + j, u);
+          nR[j] = grow(y0 + kb + kNfK + 1 + (R - 1) + j, u);
+          const int k = kb + kNfK + j;
+          nt[j] = k < h ? tp[(size_t)k * W + u] : mk<T>(0, 0);
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < kNfK; ++kk)
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const cpx<T> sm = add(L[i - kk + kNfK - 1], Rw[i + kk]);
+          accv[i] = mk<T>(accv[i].x + tk[kk].x * sm.x, accv[i].y + tk[kk].x * sm.y);
+          accd[i] = mk<T>(accd[i].x + tk[kk].y * sm.x, accd[i].y + tk[kk].y * sm.y);
+        }
+      if (more) {
+        // slide: L moves up kNfK rows, Rw down kNfK rows
+#pragma unroll
+        for (int j = NW - 1; j >= kNfK; --j) L[j] = L[j - kNfK];
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) L[j] = nL[j];
+#pragma unroll
+        for (int j = 0; j + kNfK < NW; ++j) Rw[j] = Rw[j + kNfK];
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) Rw[R - 1 + j] = nR[j];
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) tk[j] = nt[j];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const cpx<T> a = mul(accv[i], ph), b = mul(accd[i], ph);
+      cpx<T>* buf = bufs + i * Gm::LDS;
+      if (u == 0) {
+        buf[pad16(0)] = mk<T>(a.x, b.x);
+      } else {
+        buf[pad16(u)] = mk<T>(a.x - b.y, a.y + b.x);
+        buf[pad16(N - u)] = mk<T>(a.x + b.y, b.x - a.y);
+      }
+    }
+  }
+  if (t < R) bufs[t * Gm::LDS + pad16(N / 2)] = mk<T>(0, 0);  // Nyquist column
+  __syncthreads();
+  const int g = t / G, tl = t % G, y = y0 + g;
+  cpx<T>* buf = bufs + g * Gm::LDS;
+  ct_fft<T, N, true>(buf, lo, hi, tl);
+  if (y >= Ny) return;  // after the last barrier
+  const T sh = shift ? (T)*shift : (T)0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int x = tl + j * G;
+    const cpx<T> z = buf[pad16(x)];
+    out[tile_offset(x, y, N)] = mk<T>(z.x + sh, z.y);
+  }
+}
+
 // The map's row spectrum with the compile-time FFT of M = Nx/2 points (one
 // row per workgroup): the row minus its own FP64 mean, packed as z[n] =
 // x[2n] + i x[2n+1], one forward DFT_M, then the split into the half
@@ -655,10 +773,12 @@ __global__ __launch_bounds__(1024) void k_profile_many(const NodeJob* __restrict
 //   a[k][u] = rho[h+k][u] + rho[h-1-k][u],
 //   rho[s][u] = sum_c P[s][c] cos(pi u (2(c-h)+1)/Nx),
 // from the table cosx[m] = cos(pi m/Nx), m < 2 Nx.  FP64.  grid (ceil(W/256), h).
+template <class T>
 __device__ __forceinline__ void kernel_cos_at(const double* __restrict__ prof,
                                               const double* __restrict__ dprof, int n, int Nx, int W,
                                               const double* __restrict__ cosx, double* __restrict__ av,
-                                              double* __restrict__ ad, int u, int k) {
+                                              double* __restrict__ ad, cpx<T>* __restrict__ tp, int u,
+                                              int k) {
   const int h = n >> 1;
   const int s1 = h + k, s2 = h - 1 - k, twoN = 2 * Nx;
   double sa = 0.0, sd = 0.0;
@@ -674,28 +794,35 @@ __device__ __forceinline__ void kernel_cos_at(const double* __restrict__ prof,
   }
   av[(size_t)k * W + u] = sa;
   if (ad) ad[(size_t)k * W + u] = sd;
+  // (value, derivative) taps in the working precision for k_node_fused
+  if (tp) tp[(size_t)k * W + u] = mk<T>((T)sa, (T)sd);
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ prof,
                                                     const double* __restrict__ dprof, int n, int Nx,
                                                     int W, const double* __restrict__ cosx,
-                                                    double* __restrict__ av, double* __restrict__ ad) {
+                                                    double* __restrict__ av, double* __restrict__ ad,
+                                                    cpx<T>* __restrict__ tp) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= W) return;
-  kernel_cos_at(prof, dprof, n, Nx, W, cosx, av, ad, u, blockIdx.y);
+  kernel_cos_at<T>(prof, dprof, n, Nx, W, cosx, av, ad, tp, u, blockIdx.y);
 }
 
 // grid (ceil(W/256), max h, jobs).
+template <class T>
 __global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restrict__ jobs,
                                                          const double* __restrict__ prof,
                                                          const double* __restrict__ dprof, int Nx,
                                                          int W, const double* __restrict__ cosx,
                                                          double* __restrict__ av,
-                                                         double* __restrict__ ad) {
+                                                         double* __restrict__ ad,
+                                                         cpx<T>* __restrict__ tp) {
   const NodeJob& J = jobs[blockIdx.z];
   const int n = J.P.n, u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y;
   if (u >= W || k >= (n >> 1)) return;
-  kernel_cos_at(prof + J.off_p, dprof + J.off_p, n, Nx, W, cosx, av + J.off_a, ad + J.off_a, u, k);
+  kernel_cos_at<T>(prof + J.off_p, dprof + J.off_p, n, Nx, W, cosx, av + J.off_a, ad + J.off_a,
+                   tp + J.off_a, u, k);
 }
 
 // Full Ny x Nx wrapped kernel (Kernel::setKernel), for the plain path.
@@ -1024,6 +1151,58 @@ bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx,
   return done;
 }
 
+// Node map from the row spectrum by k_node_fused; false where the separate
+// y-filter and row kernels must run: widths outside 256 .. 16384, a row that
+// does not fit in LDS, fewer than h + 16 rows, or taps that the y-filter
+// would not stage either (so both paths take the column FFTs for the same
+// profiles).  Rows per workgroup: 4 up to 1024 columns, else 2 where two
+// row buffers fit, else 1.  At 4096 FP32 columns two rows (70 KB of LDS, two
+// workgroups per CU, one filtering while the other transforms) beat four
+// rows (140 KB, one workgroup per CU, fewer halo and tap reads but no
+// overlap): 0.97 against 1.25 ms for the nine node maps of a step.
+template <class T>
+bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, const cpx<T>* tp, int h, T* out, int Nx, int Ny,
+                const double* shift) {
+  const int hp = (h + kYfK - 1) / kYfK * kYfK;
+  if (sizeof(cpx<T>) * (size_t)(kYfRows + 3 * hp) * kYfCols > kLdsBytes) return false;
+  if (Ny < h + 16) return false;
+  bool done = false;
+  auto go = [&](auto nc) {
+    constexpr int N = decltype(nc)::value;
+    using Gm = CtGeom<N>;
+    auto lds_for = [](int r) { return sizeof(cpx<T>) * (size_t)(Gm::NLO + Gm::NHI + r * Gm::LDS); };
+    auto run = [&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      const size_t lds = lds_for(R);
+      auto k = k_node_fused<T, N, R>;
+      allow_lds(k, lds);
+      const int nq = (Ny + R - 1) / R;
+      MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(Gm::G * R), lds, Gs, tp, h, hp,
+                 cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
+      done = true;
+    };
+    using std::integral_constant;
+    if constexpr (N <= 1024) {
+      run(integral_constant<int, 4>{});
+    } else {
+      if (lds_for(2) <= kLdsBytes) run(integral_constant<int, 2>{});
+      else if (lds_for(1) <= kLdsBytes) run(integral_constant<int, 1>{});
+    }
+  };
+  using std::integral_constant;
+  switch (Nx) {
+    case 256: go(integral_constant<int, 256>{}); break;
+    case 512: go(integral_constant<int, 512>{}); break;
+    case 1024: go(integral_constant<int, 1024>{}); break;
+    case 2048: go(integral_constant<int, 2048>{}); break;
+    case 4096: go(integral_constant<int, 4096>{}); break;
+    case 8192: go(integral_constant<int, 8192>{}); break;
+    case 16384: go(integral_constant<int, 16384>{}); break;
+    default: break;
+  }
+  return done;
+}
+
 // y-filter node spectra from the row spectrum; false when the staged rows
 // do not fit in LDS (the caller then takes the column-FFT path).
 template <class T>
@@ -1088,12 +1267,16 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
   Workspace ws(ctx);
   int ip = ws.add(sizeof(double) * n * n), idp = ws.add(sizeof(double) * n * n);
   int ia = ws.add(sizeof(double) * h * W), iad = ws.add(sizeof(double) * h * W);
+  int itp = ws.add(sizeof(cpx<T>) * h * W);
   ws.commit();
   double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
   double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
+  cpx<T>* tp = ws.get<cpx<T>>(itp);
   MLC_LAUNCH(ctx, "profile", k_profile, dim3(1), dim3(1024), 0, P, prof, dprof);
-  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos, dim3(cdiv(W, 256), h), dim3(256), 0, prof, dprof, n,
-             m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr);
+  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos<T>, dim3(cdiv(W, 256), h), dim3(256), 0, prof, dprof, n,
+             m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr, want_d ? tp : nullptr);
+  if (d_node && !d_value && node_fused<T>(ctx, (const cpx<T>*)m->rows, tp, h, (T*)d_node, m->Nx, m->Ny, m->mean))
+    return;
   cpx<T>* G0 = (cpx<T>*)m->g0;
   cpx<T>* G1 = (cpx<T>*)m->g1;
   if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny,
@@ -1127,18 +1310,23 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
   Workspace ws(ctx);
   int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
   int ia = ws.add(sizeof(double) * std::max<int64_t>(na, 1)), iad = ws.add(sizeof(double) * std::max<int64_t>(na, 1));
+  int itp = ws.add(sizeof(cpx<T>) * std::max<int64_t>(na, 1));
   ws.commit();
   const NodeJob* dj =
       static_cast<const NodeJob*>(const_table(ctx, kTableNodeJobs, jobs.data(), sizeof(NodeJob) * cnt));
   double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
   double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
+  cpx<T>* tp = ws.get<cpx<T>>(itp);
   MLC_LAUNCH(ctx, "profile", k_profile_many, dim3(cnt), dim3(1024), 0, dj, prof, dprof);
-  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many, dim3(cdiv(W, 256), hmax, cnt), dim3(256), 0, dj,
-             prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad);
+  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many<T>, dim3(cdiv(W, 256), hmax, cnt), dim3(256), 0, dj,
+             prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad, tp);
   cpx<T>* G0 = (cpx<T>*)m->g0;
   cpx<T>* G1 = (cpx<T>*)m->g1;
   for (int k = 0; k < cnt; ++k) {
     const int h = Ps[k].n / 2;
+    if (node_fused<T>(ctx, (const cpx<T>*)m->rows, tp + jobs[k].off_a, h, (T*)d_nodes[k], m->Nx, m->Ny,
+                      m->mean))
+      continue;
     const double *a = av + jobs[k].off_a, *b = ad + jobs[k].off_a;
     if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, a, b, h, G0, G1, m->Nx, m->Ny, W)) {
       ensure_full<T>(m);

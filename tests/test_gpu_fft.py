@@ -148,6 +148,40 @@ def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec
     assert np.max(np.abs(nd[..., 1] - dref)) < tol * 20 * max(np.max(np.abs(ref)), 1.0)
 
 
+@pytest.mark.parametrize("Ny,Nx,shape,rhalf", [(64, 1024, "gaussian", 6.0),
+                                               (40, 2048, "uniform", 3.0),
+                                               (100, 256, "wavy", 3.0),
+                                               (64, 8192, "exponential", 2.0),    # 2 rows (FP32) / 1 row per block
+                                               (48, 16384, "uniform", 2.0),       # FP32 one row per block
+                                               (512, 4096, "gaussian", 8.0),
+                                               (20, 1024, "gaussian", 4.0)])      # too few rows: separate kernels
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_node_only_request_fused_kernel(ctx, orc, Ny, Nx, shape, rhalf, prec):
+    """A node map requested alone runs the y-filter and the row FFT in one
+    kernel (k_node_fused); it matches the oracle's FFT convolutions within
+    the usual tolerance and the node map of a request that also wants the
+    value map (separate y-filter and row kernels): bit for bit where a
+    workgroup holds 2 or 4 rows, within 4 ulp of the map's scale where it
+    holds one (FP32 16384 and FP64 8192 columns, which round differently)."""
+    rng = np.random.default_rng(Ny * 3 + Nx)
+    m = _smooth_map(rng, Ny, Nx)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
+    alone = spec.convolve(shape, rhalf, 1.0, value=False, node=True)
+    _, both = spec.convolve(shape, rhalf, 1.0, value=True, node=True)
+    if (prec, Nx) in (("f32", 16384), ("f64", 8192)):
+        eps = 2.0 ** -23 if prec == "f32" else 2.0 ** -52
+        assert float((alone - both).abs().max()) <= 4 * eps * float(both.abs().max())
+    else:
+        assert torch.equal(alone, both)
+    prof, dprof = orc.profile(shape, rhalf, 1.0, deriv=True)
+    ref = orc.fft_convolve(m, orc.set_kernel(prof, Nx, Ny))
+    dref = orc.fft_convolve(m, orc.set_kernel(dprof, Nx, Ny))
+    tol = 1e-12 if prec == "f64" else 2e-6
+    nd = alone.cpu().numpy().astype(np.float64)
+    assert _rel(nd[..., 0], ref) < tol
+    assert np.max(np.abs(nd[..., 1] - dref)) < tol * 20 * max(np.max(np.abs(ref)), 1.0)
+
+
 @pytest.mark.parametrize("Ny,Nx", [(128, 160), (128, 256)])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_convolve_nodes_equals_one_by_one(ctx, Ny, Nx, prec):
