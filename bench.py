@@ -92,6 +92,7 @@ def algorithmic_bytes(name, esz, ncurves):
     return {
         "conv_node_col": spec + 2 * spec,              # read map spectrum, write 2 spectra
         "conv_node_yfilter": spec + 2 * spec,          # read row spectrum, write 2 spectra
+        "conv_node_fused": spec + 2 * real,            # read row spectrum, write (mu, dmu/dr)
         "fft_row_c2r_node": 2 * spec + 2 * real,       # read 2 spectra, write (mu, dmu/dr)
         "fft_row_r2c": N * N * 8 + spec,               # read FP64 map, write spectrum
         "lc_chi2_grad": curves * EPOCHS * 4 * 2 * esz + curves * (64 + 2 * 64),

@@ -86,9 +86,10 @@ int mlc_map_create(mlc_ctx* ctx, int precision, const double* d_map, int Nx, int
     m->Nx = Nx;
     m->Ny = Ny;
     m->W = Nx / 2 + 1;
+    m->ldr = m->W + (m->W & 1);
     m->subtract_mean = subtract_mean != 0;
     size_t spec = 2 * elem(precision) * (size_t)Ny * m->W;
-    hip_check(hipMalloc(&m->rows, spec), "hipMalloc(row spectrum)");
+    hip_check(hipMalloc(&m->rows, 2 * elem(precision) * (size_t)Ny * m->ldr), "hipMalloc(row spectrum)");
     hip_check(hipMalloc(&m->fmap, spec), "hipMalloc(map spectrum)");
     hip_check(hipMalloc(&m->g0, spec), "hipMalloc(column scratch)");
     hip_check(hipMalloc(&m->g1, spec), "hipMalloc(column scratch)");

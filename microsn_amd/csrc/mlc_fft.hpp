@@ -13,8 +13,9 @@ struct mlc_map {
   mlc_ctx* ctx = nullptr;
   int precision = MLC_F32;
   int Nx = 0, Ny = 0, W = 0;
+  int ldr = 0;             // row stride of rows: W rounded up to even (16-byte aligned column pairs)
   int subtract_mean = 0;
-  void* rows = nullptr;    // Ny x W complex (precision): R2C of every row of map - mean
+  void* rows = nullptr;    // Ny x ldr complex (precision): R2C of every row of map - mean
   void* fmap = nullptr;    // Ny x W complex: 2-D spectrum / (Nx*Ny), built on first use
   bool full_ready = false; // fmap matches rows
   double* mean = nullptr;  // device FP64: map mean (0 when not subtracted)

