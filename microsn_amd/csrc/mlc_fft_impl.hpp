@@ -560,7 +560,7 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
 // arithmetic.  Each thread puts Z[u] = Hv + i Hd and Z[N-u] into the R row
 // buffers in LDS; then group g transforms row y0+g and writes it into the
 // tiled node map.  Taps tp are the (value, derivative) pairs in the working
-// precision, zero at k >= h; the sums, their order (k ascending) and the
+// precision, hq rows with zeros at k >= h; the sums, their order (k ascending) and the
 // final products are those of k_node_yfilter and k_node_rows.  Workgroups
 // run XCD by XCD over consecutive row blocks, so the halo rows of G are
 // shared in one L2.
@@ -590,15 +590,12 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
   const int y0 = q * R, t = threadIdx.x;
   for (int i = t; i < Gm::NLO + Gm::NHI; i += NT) lo[i] = tw[i];
   const double sc = 0.5 / (double)N;
-  const P2 zero{mk<T>(0, 0), mk<T>(0, 0)};
   // the pair at row r of the spectrum, r in [-Ny, 2 Ny); the pair of taps k
   auto gpair = [&](int r, int u) -> P2 {
     r = r < 0 ? r + Ny : (r >= Ny ? r - Ny : r);
     return *reinterpret_cast<const P2*>(Gs + (size_t)r * ldg + u);
   };
-  auto tpair = [&](int k, int u) -> P2 {
-    return k < h ? *reinterpret_cast<const P2*>(tp + (size_t)k * ldt + u) : zero;
-  };
+  auto tpair = [&](int k, int u) -> P2 { return *reinterpret_cast<const P2*>(tp + (size_t)k * ldt + u); };
 #pragma unroll
   for (int c = 0; c < PPT; ++c) {
     const int u = 2 * (t + c * NT);
@@ -831,12 +828,16 @@ __global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ p
                                                     int W, const double* __restrict__ cosx,
                                                     double* __restrict__ av, double* __restrict__ ad,
                                                     cpx<T>* __restrict__ tp, int ldt) {
-  const int u = blockIdx.x * 256 + threadIdx.x;
+  const int u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y;
   if (u >= W) return;
-  kernel_cos_at<T>(prof, dprof, n, Nx, W, cosx, av, ad, tp, ldt, u, blockIdx.y);
+  if (k >= (n >> 1)) {  // the zero taps up to a multiple of kNfK
+    if (tp) tp[(size_t)k * ldt + u] = mk<T>(0, 0);
+    return;
+  }
+  kernel_cos_at<T>(prof, dprof, n, Nx, W, cosx, av, ad, tp, ldt, u, k);
 }
 
-// grid (ceil(W/256), max h, jobs).
+// grid (ceil(W/256), max h rounded up to kNfK, jobs).
 template <class T>
 __global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restrict__ jobs,
                                                          const double* __restrict__ prof,
@@ -846,8 +847,12 @@ __global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restri
                                                          double* __restrict__ ad,
                                                          cpx<T>* __restrict__ tp, int ldt) {
   const NodeJob& J = jobs[blockIdx.z];
-  const int n = J.P.n, u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y;
-  if (u >= W || k >= (n >> 1)) return;
+  const int n = J.P.n, u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, h = n >> 1;
+  if (u >= W) return;
+  if (k >= h) {  // the zero taps up to a multiple of kNfK
+    if (k < (h + kNfK - 1) / kNfK * kNfK) tp[J.off_t + (size_t)k * ldt + u] = mk<T>(0, 0);
+    return;
+  }
   kernel_cos_at<T>(prof + J.off_p, dprof + J.off_p, n, Nx, W, cosx, av + J.off_a, ad + J.off_a,
                    tp + J.off_t, ldt, u, k);
 }
@@ -1296,14 +1301,15 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
   Workspace ws(ctx);
   int ip = ws.add(sizeof(double) * n * n), idp = ws.add(sizeof(double) * n * n);
   int ia = ws.add(sizeof(double) * h * W), iad = ws.add(sizeof(double) * h * W);
-  const int ldt = W + (W & 1);
-  int itp = ws.add(sizeof(cpx<T>) * h * ldt);
+  const int ldt = W + (W & 1), hq = (h + kNfK - 1) / kNfK * kNfK;
+  int itp = ws.add(sizeof(cpx<T>) * hq * ldt);
   ws.commit();
   double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
   double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
   cpx<T>* tp = ws.get<cpx<T>>(itp);
   MLC_LAUNCH(ctx, "profile", k_profile, dim3(1), dim3(1024), 0, P, prof, dprof);
-  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos<T>, dim3(cdiv(W, 256), h), dim3(256), 0, prof, dprof, n,
+  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos<T>, dim3(cdiv(W, 256), want_d ? hq : h), dim3(256), 0, prof,
+             dprof, n,
              m->Nx, W, cos_table(ctx, 1, m->Nx), av, want_d ? ad : nullptr, want_d ? tp : nullptr, ldt);
   if (d_node && !d_value &&
       node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, tp, ldt, h, (T*)d_node, m->Nx, m->Ny, m->mean))
@@ -1334,11 +1340,12 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
   int hmax = 1;
   for (int k = 0; k < cnt; ++k) {
     const int n = Ps[k].n, h = n / 2;
+    const int hq = (h + kNfK - 1) / kNfK * kNfK;
     jobs[k] = NodeJob{Ps[k], np, na, nt};
     np += (int64_t)n * n;
     na += (int64_t)h * W;
-    nt += (int64_t)h * ldt;
-    hmax = std::max(hmax, h);
+    nt += (int64_t)hq * ldt;
+    hmax = std::max(hmax, hq);
   }
   Workspace ws(ctx);
   int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
