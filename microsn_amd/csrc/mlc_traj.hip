@@ -292,6 +292,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_fill(PartGeom g, int n, c
 struct alignas(32) Epoch {
   double t, m, is, pad;
 };
+constexpr int kStageEpochs = 128;  // epochs per band a wavefront stages in LDS
 
 // The epoch records, and per (observation set o, band b) segment the
 // earliest and latest epoch time (NaN when the segment holds a NaN time).
@@ -391,6 +392,8 @@ __global__ __launch_bounds__(256) void k_lc_band(
   constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
   __shared__ double srad[ONE ? 1 : kMaxBandNodes], sih[ONE ? 1 : kMaxBandNodes];
   __shared__ LnEntry lnt[kF32 ? kLnTab : 1];
+  // per wavefront: the band's epoch records of one observation set (st_o)
+  __shared__ Epoch sep[4][kStageEpochs];
   if constexpr (!ONE) {
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -421,6 +424,8 @@ __global__ __launch_bounds__(256) void k_lc_band(
   const int xcd = blockIdx.x & 7, nwx = (gridDim.x >> 3) * 4;
   const int wx = (blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);
   const int g0 = (int)((int64_t)nit * xcd / 8), g1 = (int)((int64_t)nit * (xcd + 1) / 8);
+  Epoch* const wsep = sep[threadIdx.x >> 6];
+  int st_o = -1;  // uniform per wave
   for (int w = g0 + wx; w < g1; w += nwx) {  // uniform per wave
     const int i = w * GPW + gi;
     const bool act = i < ncur;
@@ -548,15 +553,34 @@ __global__ __launch_bounds__(256) void k_lc_band(
       s[6] = fma(tdc, mr, s[6]);
       s[7] += q;
     };
-    // epochs of this lane: j = e0 + li + kLanes k, each record read one
-    // iteration ahead
+    // the first group's observation set, staged in LDS when it changes and
+    // its band has at most kStageEpochs epochs: the epoch records then come
+    // from LDS instead of vector-memory loads
+    const int cand = __builtin_amdgcn_readfirstlane(act ? R.o : -1);
+    if (cand >= 0 && cand < nobs && cand != st_o) {
+      const int c0 = obs_off[(size_t)cand * B + b], c1 = obs_off[(size_t)cand * B + b + 1];
+      if (c0 >= 0 && c1 <= nepoch && c0 <= c1 && c1 - c0 <= kStageEpochs) {
+        for (int q = lane; q < c1 - c0; q += 64) wsep[q] = ep[c0 + q];
+        st_o = cand;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // epochs of this lane: j = e0 + li + kLanes k
     const int jend = ok ? e1 : e0;
     int j = e0 + li;
-    Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
-    for (; j < jend; j += kLanes) {
-      const Epoch E = en;
-      if (j + kLanes < jend) en = ep[j + kLanes];
-      sample(E.t, E.m, E.is);
+    if (__all(!ok || R.o == st_o)) {
+      for (; j < jend; j += kLanes) {
+        const Epoch E = wsep[j - e0];
+        sample(E.t, E.m, E.is);
+      }
+    } else {
+      // each record read one iteration ahead
+      Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
+      for (; j < jend; j += kLanes) {
+        const Epoch E = en;
+        if (j + kLanes < jend) en = ep[j + kLanes];
+        sample(E.t, E.m, E.is);
+      }
     }
     const double val = group_sum8<kLanes>(s, li);
     const unsigned long long bm = __ballot(flag);

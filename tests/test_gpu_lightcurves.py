@@ -121,6 +121,34 @@ def test_f32_matches_oracle_on_f32_maps(ctx, orc):
     assert np.max(np.abs(g[fin] - rg[fin]) / scale) < 1e-4
 
 
+@pytest.mark.parametrize("ne", [100, 128, 129, 300])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_shared_observation_set_matches_oracle(ctx, orc, prec, ne):
+    """Every curve on one observation set: a band of at most 128 epochs is
+    staged in LDS once per wavefront and read from there, a longer band from
+    global memory; chi2 and gradient against the oracle on the same maps,
+    including a curve that leaves the map (+inf) between staged curves."""
+    rng = np.random.default_rng(ne)
+    m, spec, nodes, _, _ = _setup(ctx, prec, rng)
+    B = nodes.B
+    sets = [[(np.sort(rng.uniform(0, 30, ne)), rng.normal(-1.7, 0.3, ne), rng.uniform(0.03, 0.08, ne))
+             for _ in range(B)]]
+    obs = lcm.Observations(ctx, B, sets)
+    prm, prof, obsi = _params(rng, 600, 128, 160, nodes.P, 1)
+    prm[37, 0] = -5.0
+    c, g = lcm.chi2_grad(ctx, nodes, obs, torch.tensor(prm, device="cuda"), torch.tensor(prof, device="cuda"),
+                         torch.tensor(obsi, device="cuda"))
+    rc, rg = _oracle(orc, nodes, sets, prm, prof, obsi)
+    c, g = c.cpu().numpy(), g.cpu().numpy()
+    fin = np.isfinite(rc)
+    assert not fin[37] and fin.sum() > 500
+    assert np.array_equal(np.isfinite(c), fin)
+    tc, tg = (1e-11, 1e-10) if prec == "f64" else (1e-5, 1e-4)
+    assert np.max(np.abs(c[fin] - rc[fin]) / np.abs(rc[fin])) < tc
+    scale = np.maximum(np.abs(rg[fin]).max(axis=1, keepdims=True), 1e-30)
+    assert np.max(np.abs(g[fin] - rg[fin]) / scale) < tg
+
+
 def test_adjoint_matches_central_differences(ctx):
     rng = np.random.default_rng(12)
     m, spec, nodes, obs, sets = _setup(ctx, "f64", rng)
