@@ -560,10 +560,10 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
 // arithmetic.  Each thread puts Z[u] = Hv + i Hd and Z[N-u] into the R row
 // buffers in LDS; then group g transforms row y0+g and writes it into the
 // tiled node map.  Taps tp are the (value, derivative) pairs in the working
-// precision, hq rows with zeros at k >= h; the sums, their order (k ascending) and the
-// final products are those of k_node_yfilter and k_node_rows.  Workgroups
-// run XCD by XCD over consecutive row blocks, so the halo rows of G are
-// shared in one L2.
+// precision, hq rows (h rounded up to a multiple of kNfK, zeros at k >= h);
+// the sums, their order (k ascending) and the final products are those of
+// k_node_yfilter and k_node_rows.  Workgroups run XCD by XCD over
+// consecutive row blocks, so the halo rows of G are shared in one L2.
 constexpr int kNfK = 2;
 
 template <class T>
