@@ -1,5 +1,5 @@
 // mlc_traj.hip -- per-light-curve magnitude chi2 and its adjoint along
-// straight trajectories (the north-star generalisation of kernel_sample,
+// straight trajectories (a generalisation of kernel_sample,
 // gpu_functions.cu:604-620, and of the chi2 of kernel_chi2, 700-750).
 //
 // A curve c has parameters (x0, y0, theta, v, r0, w, dm), a profile p and an

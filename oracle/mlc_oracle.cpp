@@ -102,7 +102,7 @@ static void shape_value(int shape, double r, double rhalf, const double* par,
   double v = 0.0, dv = 0.0;
   switch (shape) {
     case ORC_UNIFORM:  // profile.cpp:211-218.  The indicator's derivative is
-                       // taken as that of a one-pixel linear edge ramp (north-star
+                       // taken as that of a one-pixel linear edge ramp (light-curve
                        // model convention): dR/drhalf / pix inside |r - R| < pix/2.
       v = (r < par[0]) ? 1.0 : 0.0;
       if (std::fabs(r - par[0]) < 0.5 * pix_of_grid) dv = std::sqrt(2.0) / pix_of_grid;
@@ -718,7 +718,7 @@ void orc_chi2_multiband(const float* f_obs, const float* df_obs, const float* mu
 }
 
 // ---------------------------------------------------------------------------
-// north-star model
+// light-curve model (no reference function)
 double orc_bilinear(const double* map, int Nx, int Ny, double x, double y) {
   if (!(x >= 0.0 && x <= Nx - 1 && y >= 0.0 && y <= Ny - 1)) return 0.0;
   int x0 = (int)std::floor(x), y0 = (int)std::floor(y);

@@ -8,7 +8,7 @@
  *
  * Every function cites the reference file:line it follows.  Functions that
  * have no counterpart in the reference (the trajectory/magnitude chi2 and its
- * adjoint) are marked "north-star model" and are pinned by closed forms,
+ * adjoint) are marked "light-curve model" and are pinned by closed forms,
  * brute force and central finite differences in tests/.
  */
 #ifndef MLC_ORACLE_H
@@ -48,7 +48,7 @@ int orc_profile_values(int shape, double rhalf, double pix, double extra_rin,
                        double extra_a, double* out, int n_cap);
 
 /* Discretised kernel and its derivative with respect to the half-light radius
- * at a fixed grid size n (north-star model; the grid size is the reference's
+ * at a fixed grid size n (light-curve model; the grid size is the reference's
  * n(rhalf)).  dout may be NULL. */
 int orc_profile_values_deriv(int shape, double rhalf, double pix,
                              double extra_rin, double extra_a, double* out,
@@ -153,7 +153,7 @@ void orc_chi2_multiband(const float* f_obs, const float* df_obs,
                         int gpu_quirk, float* chi2_all, float* likelihood_f32,
                         double* likelihood_f64);
 
-/* ---- north-star model: trajectories, magnitudes, chi2 and adjoint ----
+/* ---- light-curve model: trajectories, magnitudes, chi2 and adjoint ----
  * See microsn_amd/csrc/mlc_traj.hip for the GPU version; this is the FP64
  * definition.
  *

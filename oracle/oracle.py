@@ -317,7 +317,7 @@ def chi2_multiband(f_obs, df_obs, mu1, mu2, gpu_quirk=False, want_all=True):
     return chi2_all, float(l32[0]), float(l64[0])
 
 
-# ---------------------------------------------------------------- north star
+# ---------------------------------------------------------------- light-curve model
 def bilinear(m, x, y):
     m = f64(m)
     return lib().orc_bilinear(_p(m), m.shape[1], m.shape[0], x, y)

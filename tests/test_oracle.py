@@ -368,7 +368,7 @@ def test_multiband_known_answer():
     assert abs(l64 - 4508.1) < 0.1
 
 
-# ---------------------------------------------------------------- light curves (north star)
+# ---------------------------------------------------------------- light curves (magnitude chi2 model)
 def test_bilinear_point_samples_and_affine_exactness():
     rng = np.random.default_rng(12)
     m = rng.random((9, 11))
