@@ -10,6 +10,12 @@
  * have no counterpart in the reference (the trajectory/magnitude chi2 and its
  * adjoint) are marked "light-curve model" and are pinned by closed forms,
  * brute force and central finite differences in tests/.
+ *
+ * Parity unpinned: the d/drhalf of the uniform disc (a one-pixel edge ramp,
+ * a model convention with no reference counterpart), and the values of the
+ * thermal_hole and wavy profiles beyond their normalisation, symmetry,
+ * zero regions and amplitude invariance (their d/drhalf is pinned by
+ * central differences like the other shapes).
  */
 #ifndef MLC_ORACLE_H
 #define MLC_ORACLE_H
