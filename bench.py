@@ -353,13 +353,20 @@ def main():
         torch.cuda.empty_cache()
 
     # ---- roofline of the dominant kernel
-    dom = max((k for k in kern if algorithmic_bytes(k, 4, ncurves)), key=lambda k: kern[k][0])
-    ms_tot, launches = kern[dom]
-    avg_s = ms_tot / launches / 1e3
-    ab = algorithmic_bytes(dom, 4, ncurves)
-    roof = {"kernel": dom, "bound": "hbm", "achieved": ab / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": measured_traffic(dom),
-            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": avg_s * 1e3}
+    def roofline(k):
+        ms_tot, launches = kern[k]
+        avg_s = ms_tot / launches / 1e3
+        ab = algorithmic_bytes(k, 4, ncurves)
+        return {"kernel": k, "bound": "hbm", "achieved": ab / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": measured_traffic(k),
+                "algorithmic_bytes_per_launch": ab, "avg_launch_ms": avg_s * 1e3,
+                "ms_per_step": ms_tot / max(1, ksteps)}
+
+    # the dominant kernel by time per step, and every kernel with algorithmic bytes
+    timed = [k for k in kern if algorithmic_bytes(k, 4, ncurves)]
+    dom = max(timed, key=lambda k: kern[k][0])
+    roof = roofline(dom)
+    roof_all = {k: roofline(k) for k in sorted(timed, key=lambda k: -kern[k][0])}
 
     cpu = None
     if not args.no_cpu:
@@ -378,7 +385,7 @@ def main():
                    "map": N, "curves": ncurves, "epochs_per_curve": EPOCHS * len(BAND_SCALE),
                    "nodes": len(SHAPES) * len(BAND_SCALE)},
         "fp64_plain": plain, "speedup_vs_fp64_plain": (ncurves / dt32) / plain["value"] if plain else None,
-        "precision": err, "roofline": roof, "cpu_baseline": cpu,
+        "precision": err, "roofline": roof, "roofline_by_kernel": roof_all, "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v[0] / ksteps for k, v in sorted(kern.items())},
         "ms_per_step_with_kernel_events": dt32_ev * 1e3,
         "hip_graph": graph_info,
