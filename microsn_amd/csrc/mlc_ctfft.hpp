@@ -9,7 +9,10 @@
 // (the map of mlc_fft_core.hpp).  Each stage reads all of its inputs into
 // registers, synchronises, then writes, so one buffer serves in place.
 // Twiddles w^m = exp(-+2 pi i m/N) come from two short tables, w^m =
-// hi[m >> LB] * lo[m & (2^LB - 1)], one complex product per twiddle.
+// hi[m >> LB] * lo[m & (2^LB - 1)], one complex product per twiddle, except
+// in the second stage (Ns = 16), whose 256 twiddles w^(k r N/256), k, r <
+// 16, are a table of their own stored right after hi (t2[16 r + k], so the
+// 16 values of k read together lie in consecutive banks).
 // LDS indices are padded by one element per 16 (pad16) so the strided
 // writes of the first stages do not pile onto a few banks.
 #pragma once
@@ -26,6 +29,7 @@ struct CtGeom {
   static constexpr int G = N / 16;           // threads per transform
   static constexpr int LB = ct_log2(N) / 2;  // bits of the low twiddle table
   static constexpr int NLO = 1 << LB, NHI = N >> LB;
+  static constexpr int NT2 = 256, NTW = NLO + NHI + NT2;  // lo | hi | t2
   static constexpr int LDS = N + N / 16;     // padded buffer length (elements)
 };
 
@@ -58,7 +62,11 @@ struct CtStage {
       const int j = t + b * Gm::G;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(j + r * NB)];
-      if constexpr (NS > 1) {
+      if constexpr (NS == 16) {
+        const cpx<T>* t2 = hi + Gm::NHI + (j & 15);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = mul(v[b][r], t2[16 * r]);
+      } else if constexpr (NS > 1) {
         const int k = j & (NS - 1);
 #pragma unroll
         for (int r = 1; r < R; ++r) {

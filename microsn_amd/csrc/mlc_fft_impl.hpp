@@ -501,11 +501,11 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* hi = lo + Gm::NLO;
-  cpx<T>* buf = hi + Gm::NHI;
+  cpx<T>* buf = lo + Gm::NTW;
   const int y = xcd_block(Ny);
   if (y >= Ny) return;
   const int t = threadIdx.x;
-  for (int i = t; i < Gm::NLO + Gm::NHI; i += G) lo[i] = tw[i];
+  for (int i = t; i < Gm::NTW; i += G) lo[i] = tw[i];
   const cpx<T>* hv = Hv + (size_t)y * W;
   const cpx<T>* hd = Hd + (size_t)y * W;
   {
@@ -583,12 +583,12 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* hi = lo + Gm::NLO;
-  cpx<T>* bufs = hi + Gm::NHI;  // R buffers of Gm::LDS elements
+  cpx<T>* bufs = lo + Gm::NTW;  // R buffers of Gm::LDS elements
   const int nq = (Ny + R - 1) / R;
   const int q = xcd_block(nq);
   if (q >= nq) return;
   const int y0 = q * R, t = threadIdx.x;
-  for (int i = t; i < Gm::NLO + Gm::NHI; i += NT) lo[i] = tw[i];
+  for (int i = t; i < Gm::NTW; i += NT) lo[i] = tw[i];
   const double sc = 0.5 / (double)N;
   // the pair at row r of the spectrum, r in [-Ny, 2 Ny); the pair of taps k
   auto gpair = [&](int r, int u) -> P2 {
@@ -703,9 +703,9 @@ __global__ __launch_bounds__(CtGeom<M>::G) void k_map_rows(const double* __restr
   __shared__ double red[(G + 63) / 64];
   cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* hi = lo + Gm::NLO;
-  cpx<T>* buf = hi + Gm::NHI;
+  cpx<T>* buf = lo + Gm::NTW;
   const int y = blockIdx.x, t = threadIdx.x;
-  for (int i = t; i < Gm::NLO + Gm::NHI; i += G) lo[i] = tw[i];
+  for (int i = t; i < Gm::NTW; i += G) lo[i] = tw[i];
   const cpx<double>* row = reinterpret_cast<const cpx<double>*>(in + (size_t)y * 2 * M);
   cpx<double> v[NU];
 #pragma unroll
@@ -1097,9 +1097,10 @@ void node_col(mlc_ctx* ctx, const cpx<T>* fmap, const double* av, const double* 
 }
 
 // Twiddle tables of mlc_ctfft.hpp for length N: lo[l] = w^l, l < 2^LB,
-// then hi[h] = w^(h 2^LB), with w = exp(+2 pi i/N) for the inverse transform
-// and exp(-2 pi i/N) for the forward one (the stages use the table as
-// given); cached per context and direction.
+// then hi[h] = w^(h 2^LB), then t2[16 r + k] = w^(k r N/256), k, r < 16,
+// with w = exp(+2 pi i/N) for the inverse transform and exp(-2 pi i/N) for
+// the forward one (the stages use the table as given), each entry rounded
+// once from long double; cached per context and direction.
 template <class T, int N, bool INV>
 const cpx<T>* ct_twiddles(mlc_ctx* ctx) {
   using Gm = CtGeom<N>;
@@ -1115,6 +1116,11 @@ const cpx<T>* ct_twiddles(mlc_ctx* ctx) {
     const long double a = sg * 2 * pi * (long double)k * Gm::NLO / N;
     h.push_back(mk<T>((T)std::cos(a), (T)std::sin(a)));
   }
+  for (int r = 0; r < 16; ++r)  // t2: the second stage's w^(k r N/256)
+    for (int k = 0; k < 16; ++k) {
+      const long double a = sg * 2 * pi * (long double)(k * r) / 256;
+      h.push_back(mk<T>((T)std::cos(a), (T)std::sin(a)));
+    }
   void* d = nullptr;
   hip_check(hipMalloc(&d, sizeof(cpx<T>) * h.size()), "hipMalloc(twiddles)");
   ctx->twiddles[key] = d;
@@ -1134,7 +1140,7 @@ bool map_rows(mlc_ctx* ctx, const double* in, cpx<T>* out, int ldo, double* rows
   auto go = [&](auto mc) {
     constexpr int M = decltype(mc)::value;
     using Gm = CtGeom<M>;
-    const size_t lds = sizeof(cpx<T>) * (size_t)(Gm::NLO + Gm::NHI + Gm::LDS);
+    const size_t lds = sizeof(cpx<T>) * (size_t)(Gm::NTW + Gm::LDS);
     if (lds > kLdsBytes - 1024) return;
     auto k = k_map_rows<T, M>;
     allow_lds(k, lds);
@@ -1162,7 +1168,7 @@ bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx,
   auto go = [&](auto nc) {
     constexpr int N = decltype(nc)::value;
     using Gm = CtGeom<N>;
-    const size_t lds = sizeof(cpx<T>) * (size_t)(Gm::NLO + Gm::NHI + Gm::LDS);
+    const size_t lds = sizeof(cpx<T>) * (size_t)(Gm::NTW + Gm::LDS);
     if (lds > kLdsBytes) return;
     auto k = k_node_rows<T, N>;
     allow_lds(k, lds);
@@ -1203,7 +1209,7 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int l
   auto go = [&](auto nc) {
     constexpr int N = decltype(nc)::value;
     using Gm = CtGeom<N>;
-    auto lds_for = [](int r) { return sizeof(cpx<T>) * (size_t)(Gm::NLO + Gm::NHI + r * Gm::LDS); };
+    auto lds_for = [](int r) { return sizeof(cpx<T>) * (size_t)(Gm::NTW + r * Gm::LDS); };
     auto run = [&](auto rc) {
       constexpr int R = decltype(rc)::value;
       const size_t lds = lds_for(R);

@@ -129,16 +129,25 @@ static void ct_host(double* data, int inverse) {
   using Gm = CtGeom<N>;
   const long double pi = 3.14159265358979323846264338327950288L;
   const long double sg = inverse ? 1.0L : -1.0L;
-  std::vector<cpx<double>> lo(Gm::NLO), hi(Gm::NHI), buf(Gm::LDS);
+  // one table as the kernels hold it: lo | hi | t2 (the second stage's)
+  std::vector<cpx<double>> tw(Gm::NTW), buf(Gm::LDS);
+  cpx<double>* lo = tw.data();
+  cpx<double>* hi = lo + Gm::NLO;
+  cpx<double>* t2 = hi + Gm::NHI;
   for (int l = 0; l < Gm::NLO; ++l)
     lo[l] = mk((double)std::cos(sg * 2 * pi * l / N), (double)std::sin(sg * 2 * pi * l / N));
   for (int h = 0; h < Gm::NHI; ++h) {
     long double a = sg * 2 * pi * (long double)h * Gm::NLO / N;
     hi[h] = mk((double)std::cos(a), (double)std::sin(a));
   }
+  for (int r = 0; r < 16; ++r)
+    for (int k = 0; k < 16; ++k) {
+      long double a = sg * 2 * pi * (long double)(k * r) / 256;
+      t2[16 * r + k] = mk((double)std::cos(a), (double)std::sin(a));
+    }
   for (int i = 0; i < N; ++i) buf[pad16(i)] = mk(data[2 * i], data[2 * i + 1]);
-  if (inverse) ct_host_stages<double, N, 1, true>(buf.data(), lo.data(), hi.data());
-  else ct_host_stages<double, N, 1, false>(buf.data(), lo.data(), hi.data());
+  if (inverse) ct_host_stages<double, N, 1, true>(buf.data(), lo, hi);
+  else ct_host_stages<double, N, 1, false>(buf.data(), lo, hi);
   for (int i = 0; i < N; ++i) {
     data[2 * i] = buf[pad16(i)].x;
     data[2 * i + 1] = buf[pad16(i)].y;
