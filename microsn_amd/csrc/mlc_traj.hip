@@ -553,11 +553,13 @@ __global__ __launch_bounds__(256) void k_lc_band(
       s[6] = fma(tdc, mr, s[6]);
       s[7] += q;
     };
-    // the first group's observation set, staged in LDS when it changes and
-    // its band has at most kStageEpochs epochs: the epoch records then come
-    // from LDS instead of vector-memory loads
+    // the first group's observation set, staged in LDS when it changes,
+    // every active group of the wavefront uses it (otherwise the copy would
+    // not save the groups' own loads) and its band has at most
+    // kStageEpochs epochs: the epoch records then come from LDS instead of
+    // vector-memory loads
     const int cand = __builtin_amdgcn_readfirstlane(act ? R.o : -1);
-    if (cand >= 0 && cand < nobs && cand != st_o) {
+    if (cand >= 0 && cand < nobs && cand != st_o && __all(!act || R.o == cand)) {
       const int c0 = obs_off[(size_t)cand * B + b], c1 = obs_off[(size_t)cand * B + b + 1];
       if (c0 >= 0 && c1 <= nepoch && c0 <= c1 && c1 - c0 <= kStageEpochs) {
         for (int q = lane; q < c1 - c0; q += 64) wsep[q] = ep[c0 + q];
