@@ -131,7 +131,11 @@ int mlc_profile_size(int shape, double rhalf, double pix, double rin, double a);
  *   d_node : Ny x Nx interleaved (value, d value / d rhalf) pairs, or NULL;
  *            stored in 4 x 4 pixel tiles: pixel (x, y) is pair number
  *            ((y/4)*(Nx/4) + x/4)*16 + (y%4)*4 + x%4 (needs Nx, Ny % 4 == 0)
- * At least one of them must be given. */
+ * At least one of them must be given.  A node map alone (d_value NULL, Nx a
+ * power of two up to 16384) runs the filter and the row FFTs in one kernel
+ * and never stores the node spectra; it agrees with the node map of a call
+ * that also asks for d_value to within a few ulp of the map's scale (the
+ * same sums in the same order, compiled into different kernels). */
 int mlc_map_convolve(mlc_map* map, int shape, double rhalf, double pix,
                      double rin, double a, void* d_value, void* d_node);
 /* Node maps of count profiles in one call: d_nodes[k] receives what
