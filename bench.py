@@ -116,9 +116,11 @@ def run_step(ctx, ws, plain=False):
     import microsn_amd.lightcurves as lcm
 
     ws["spec"].update(ws["map"])
-    ws["nodes"].compute(plain)
+    if plain:
+        ws["nodes"].compute(plain)
+    # fused path: each node map is convolved right before the band launch that reads it
     chi2, grad = lcm.chi2_grad(ctx, ws["nodes"], ws["obs"], ws["params"], ws["prof"], ws["obs_idx"],
-                               plain=plain)
+                               plain=plain, convolve=not plain)
     perm, _, sk = api.sort_f64(ctx, chi2, sorted_keys=True)
     # bin table and binned means on the device: no host round trip in a step
     lo, npb, _, be = api.bin_table_device(ctx, sk, ws["bins"])

@@ -58,7 +58,7 @@ void mlc_ctx_destroy(mlc_ctx* ctx);
 const char* mlc_last_error(const mlc_ctx* ctx);
 /* All later work of the context goes to `stream`.  Graph capture: the calls
  * of one light-curve step (mlc_map_update, mlc_map_convolve_nodes,
- * mlc_lc_chi2_grad, mlc_sort_f64, mlc_bin_table_device) issue no host
+ * mlc_lc_chi2_grad or mlc_lc_chi2_grad_convolve, mlc_sort_f64, mlc_bin_table_device) issue no host
  * synchronisation, memset node or host copy once they have run with the same
  * sizes and tables, so a second identical step can be captured on `stream`
  * and replayed.  A captured graph refers to the context's workspace and
@@ -282,6 +282,28 @@ int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes,
                      const int* d_obs, int nobs, const int* d_obs_off,
                      int nepoch, const double* d_t, const double* d_mobs,
                      const double* d_sig, double* d_chi2, double* d_grad);
+/* The node maps and the chi2 of one step in one call:
+ *   mlc_map_convolve_nodes(map, nnodes, node_shape, node_rad, pix, NULL, NULL,
+ *                          {d_nodes + k * Nx*Ny pairs})
+ * followed by mlc_lc_chi2_grad(ctx of map, precision of map, d_nodes, ...),
+ * with bit-identical node maps, chi2 and gradient, but node k is convolved
+ * right before the first (profile, band) launch that reads it, so that
+ * launch finds its maps in the last-level cache instead of HBM.  The order
+ * mirrors the per-profile convolve-then-sample loop of expanding_source
+ * (gpu_functions.cu:180-225).  d_nodes: nnodes consecutive node maps (the
+ * layout mlc_map_convolve writes to d_node); node_shape: host array of
+ * nnodes MLC_SHAPE_* values; node_rad doubles as each node's half-light
+ * radius (in units of pix) for the convolution.  Every node map is written,
+ * including those no band reads and when ncurves is 0. */
+int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix,
+                              void* d_nodes, int nnodes, int P, int B,
+                              const int* node_off, const int* node_cnt,
+                              const double* node_rad, const double* band_scale,
+                              int ncurves, const double* d_params,
+                              const int* d_prof, const int* d_obs, int nobs,
+                              const int* d_obs_off, int nepoch, const double* d_t,
+                              const double* d_mobs, const double* d_sig,
+                              double* d_chi2, double* d_grad);
 /* Same result, structured like the reference's plain pipeline: one launch
  * writes per-sample (mu, dmu/dx, dmu/dy, dmu/drad) to device memory, the
  * stream is synchronised, a second launch reduces chi2 and gradient.  The

@@ -64,6 +64,8 @@ SIGNATURES = {
     # light curves
     "mlc_lc_chi2_grad": (_i, [_p, _i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p,
                               _i, _p, _i, _p, _p, _p, _p, _p]),
+    "mlc_lc_chi2_grad_convolve": (_i, [_p, _p, _d, _p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p, _i,
+                                       _p, _i, _p, _p, _p, _p, _p]),
     "mlc_lc_chi2_grad_plain": (_i, [_p, _i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p,
                                     _p, _i, _p, _i, _p, _p, _p, _p, _p]),
     # mlc_host.h

@@ -115,6 +115,7 @@ void mlc_map_destroy(mlc_map* m) {
   (void)hipFree(m->fmap);
   (void)hipFree(m->g0);
   (void)hipFree(m->g1);
+  (void)hipFree(m->ntab);
   (void)hipFree(m->mean);
   delete m;
 }

@@ -3,6 +3,7 @@
 // (mlc_fft_f32.hip, mlc_fft_f64.hip).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "mlc_fft_core.hpp"
@@ -21,6 +22,8 @@ struct mlc_map {
   double* mean = nullptr;  // device FP64: map mean (0 when not subtracted)
   void* g0 = nullptr;      // Ny x W complex scratch (value column output)
   void* g1 = nullptr;      // Ny x W complex scratch (derivative column output)
+  void* ntab = nullptr;    // node profiles and taps of the last mlc_map_convolve_nodes call
+  size_t ntab_bytes = 0;
 };
 
 namespace mlc {
@@ -38,9 +41,14 @@ void fft_map_transform(mlc_map* m, const double* d_map);
 // Fused convolution (see mlc_map_convolve).
 template <class T>
 void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
-// Node maps of several profiles (see mlc_map_convolve_nodes).
+// Node maps of several profiles (see mlc_map_convolve_nodes).  Without a
+// driver the nodes are convolved in order; with one, drive(conv) is called
+// once after the profiles and taps of all nodes are built, and conv(k)
+// convolves node k when the driver calls it (each node once).
+using NodeDriver = std::function<void(const std::function<void(int)>& conv)>;
 template <class T>
-void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes);
+void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes,
+                        const NodeDriver* drive = nullptr);
 // Reference-structured convolution (see mlc_map_convolve_plain).
 template <class T>
 void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
@@ -54,8 +62,10 @@ extern template void fft_map_transform<float>(mlc_map*, const double*);
 extern template void fft_map_transform<double>(mlc_map*, const double*);
 extern template void fft_convolve_fused<float>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_convolve_fused<double>(mlc_map*, const ProfileParams&, void*, void*);
-extern template void fft_convolve_nodes<float>(mlc_map*, const std::vector<ProfileParams>&, void* const*);
-extern template void fft_convolve_nodes<double>(mlc_map*, const std::vector<ProfileParams>&, void* const*);
+extern template void fft_convolve_nodes<float>(mlc_map*, const std::vector<ProfileParams>&, void* const*,
+                                               const NodeDriver*);
+extern template void fft_convolve_nodes<double>(mlc_map*, const std::vector<ProfileParams>&, void* const*,
+                                                const NodeDriver*);
 extern template void fft_convolve_plain<float>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_convolve_plain<double>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_r2c_2d<float>(mlc_ctx*, const float*, cpx<float>*, int, int);

@@ -1334,12 +1334,19 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
 
 // Node maps of several profiles: the profiles and their cosine
 // coefficients in one launch each, then the y-filter and row transforms of
-// each node in turn (the same kernels and arithmetic as fft_convolve_fused).
+// each node (the same kernels and arithmetic as fft_convolve_fused), in
+// order or in the order a driver asks for them.  The profile tables live in
+// the map's own buffer, so a driver may run work that uses the context
+// workspace between the nodes.
 template <class T>
-void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes) {
+void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes,
+                        const NodeDriver* drive) {
   mlc_ctx* ctx = m->ctx;
   const int cnt = (int)Ps.size(), W = m->W;
-  if (cnt == 0) return;
+  if (cnt == 0) {
+    if (drive) (*drive)([](int) {});
+    return;
+  }
   std::vector<NodeJob> jobs(cnt);
   const int ldt = W + (W & 1);
   int64_t np = 0, na = 0, nt = 0;
@@ -1353,7 +1360,7 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
     nt += (int64_t)hq * ldt;
     hmax = std::max(hmax, hq);
   }
-  Workspace ws(ctx);
+  Workspace ws(ctx, &m->ntab, &m->ntab_bytes);
   int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
   int ia = ws.add(sizeof(double) * std::max<int64_t>(na, 1)), iad = ws.add(sizeof(double) * std::max<int64_t>(na, 1));
   int itp = ws.add(sizeof(cpx<T>) * std::max<int64_t>(nt, 1));
@@ -1368,11 +1375,12 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
              prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad, tp, ldt);
   cpx<T>* G0 = (cpx<T>*)m->g0;
   cpx<T>* G1 = (cpx<T>*)m->g1;
-  for (int k = 0; k < cnt; ++k) {
+  auto conv = [&](int k) {
+    MLC_REQUIRE(k >= 0 && k < cnt, "convolve_nodes: node index out of range");
     const int h = Ps[k].n / 2;
     if (node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, tp + jobs[k].off_t, ldt, h, (T*)d_nodes[k], m->Nx,
                       m->Ny, m->mean))
-      continue;
+      return;
     const double *a = av + jobs[k].off_a, *b = ad + jobs[k].off_a;
     if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, m->ldr, a, b, h, G0, G1, m->Nx, m->Ny, W)) {
       ensure_full<T>(m);
@@ -1380,6 +1388,11 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
     }
     if (!node_rows<T>(ctx, G0, G1, (T*)d_nodes[k], m->Nx, m->Ny, m->mean))
       row_c2r<T, 2>(ctx, G0, G1, (T*)d_nodes[k], m->Nx, m->Ny, m->mean);
+  };
+  if (drive) {
+    (*drive)(conv);
+  } else {
+    for (int k = 0; k < cnt; ++k) conv(k);
   }
 }
 
@@ -1443,7 +1456,8 @@ void fft_c2r_2d(mlc_ctx* ctx, cpx<T>* work, T* out, int Nx, int Ny) {
 #define MLC_FFT_INSTANTIATE(T)                                                               \
   template void fft_map_transform<T>(mlc_map*, const double*);                               \
   template void fft_convolve_fused<T>(mlc_map*, const ProfileParams&, void*, void*);         \
-  template void fft_convolve_nodes<T>(mlc_map*, const std::vector<ProfileParams>&, void* const*); \
+  template void fft_convolve_nodes<T>(mlc_map*, const std::vector<ProfileParams>&, void* const*,    \
+                                      const NodeDriver*);                                    \
   template void fft_convolve_plain<T>(mlc_map*, const ProfileParams&, void*, void*);         \
   template void fft_r2c_2d<T>(mlc_ctx*, const T*, cpx<T>*, int, int);                        \
   template void fft_c2r_2d<T>(mlc_ctx*, cpx<T>*, T*, int, int);

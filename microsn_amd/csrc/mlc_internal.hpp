@@ -91,13 +91,18 @@ struct DeviceGuard {
   }
 };
 
-// Slices of the context workspace.  Grows (free + malloc) when a call needs
-// more than the current size; the previous contents are not kept.
+// Slices of a growable device buffer: the context workspace by default, or
+// a buffer owned by another object (so that two live Workspaces never share
+// memory).  Grows (free + malloc) when a call needs more than the current
+// size; the previous contents are not kept.
 struct Workspace {
   mlc_ctx* ctx;
+  void** buf;
+  size_t* cap;
   size_t used = 0;
   std::vector<std::pair<size_t, size_t>> slices;  // offset, bytes
-  explicit Workspace(mlc_ctx* c) : ctx(c) {}
+  explicit Workspace(mlc_ctx* c) : ctx(c), buf(&c->ws), cap(&c->ws_bytes) {}
+  Workspace(mlc_ctx* c, void** b, size_t* n) : ctx(c), buf(b), cap(n) {}
   // Declare a slice; returns its index.  Call commit() before get().
   int add(size_t bytes) {
     size_t off = (used + 255) & ~(size_t)255;
@@ -106,21 +111,21 @@ struct Workspace {
     return (int)slices.size() - 1;
   }
   void commit() {
-    if (used > ctx->ws_bytes) {
-      if (ctx->ws) {
+    if (used > *cap) {
+      if (*buf) {
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(workspace)");
-        hip_check(hipFree(ctx->ws), "hipFree(workspace)");
-        ctx->ws = nullptr;
-        ctx->ws_bytes = 0;
+        hip_check(hipFree(*buf), "hipFree(workspace)");
+        *buf = nullptr;
+        *cap = 0;
       }
       size_t sz = used + (used >> 3) + 4096;
-      hip_check(hipMalloc(&ctx->ws, sz), "hipMalloc(workspace)");
-      ctx->ws_bytes = sz;
+      hip_check(hipMalloc(buf, sz), "hipMalloc(workspace)");
+      *cap = sz;
     }
   }
   template <class T>
   T* get(int i) const {
-    return reinterpret_cast<T*>(static_cast<char*>(ctx->ws) + slices[i].first);
+    return reinterpret_cast<T*>(static_cast<char*>(*buf) + slices[i].first);
   }
 };
 

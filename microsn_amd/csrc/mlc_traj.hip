@@ -26,8 +26,10 @@
 // depend on scheduling.  A curve that leaves the pixel-centre hull or meets
 // mu <= 0 gets chi2 = +inf and a zero gradient.
 #include <cmath>
+#include <functional>
 #include <vector>
 
+#include "mlc_fft.hpp"
 #include "mlc_fft_core.hpp"
 #include "mlc_internal.hpp"
 #include "mlc_math.hpp"
@@ -772,12 +774,18 @@ NodeTab upload_tab(mlc_ctx* ctx, const Args& a, Workspace& ws, int io, int ic, i
   return NodeTab{ws.get<int>(io), ws.get<int>(ic), ws.get<double>(ir), ws.get<double>(ib)};
 }
 
-void run_fused(mlc_ctx* ctx, const Args& a) {
+void validate_fused(const Args& a) {
   validate(a);
   MLC_REQUIRE(a.P < kMaxBuckets, "lc_chi2_grad: at most 8191 profiles");
   MLC_REQUIRE((int64_t)a.Nx * a.Ny <= INT32_MAX, "lc_chi2_grad: map too large for 32-bit offsets");
   for (int i = 0; i < a.P * a.B; ++i)
     MLC_REQUIRE(a.node_cnt[i] <= kMaxBandNodes, "lc_chi2_grad: at most 16 radius nodes per (profile, band)");
+}
+
+// before_band(p*B + b), when given, runs on the host before the launch of
+// band b of profile p is queued (the caller writes that band's node maps).
+void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* before_band = nullptr) {
+  validate_fused(a);
   if (a.ncurves == 0) return;
   const int P = a.P, B = a.B, n = a.ncurves;
   PartGeom pg{P, 1, 0, 0.0, 0.0, a.prof, a.params, a.obs, a.nobs, B, a.nepoch, a.obs_off, a.t};
@@ -835,6 +843,7 @@ void run_fused(mlc_ctx* ctx, const Args& a) {
       bn.bsc = a.band_scale[b];
       for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
       const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
+      if (before_band) (*before_band)(pb);
 #define MLC_BAND(TT, F, O)                                                                          \
   MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O>),                                            \
              dim3(grid_for(reinterpret_cast<const void*>(&k_lc_band<TT, F, O>))), dim3(256), 0,      \
@@ -905,6 +914,10 @@ void run_plain(mlc_ctx* ctx, const Args& a) {
 
 }  // namespace
 
+namespace mlc {
+ProfileParams checked_params(int shape, double rhalf, double pix, double rin, double a, int Nx, int Ny);
+}
+
 extern "C" {
 
 int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes, int nnodes, int Nx, int Ny,
@@ -917,6 +930,48 @@ int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes, int nnode
            ncurves, d_params, d_prof, d_obs, nobs, d_obs_off, nepoch, d_t, d_mobs, d_sig, d_chi2,
            d_grad};
     run_fused(ctx, a);
+  });
+}
+
+int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix, void* d_nodes, int nnodes,
+                              int P, int B, const int* node_off, const int* node_cnt,
+                              const double* node_rad, const double* band_scale, int ncurves,
+                              const double* d_params, const int* d_prof, const int* d_obs, int nobs,
+                              const int* d_obs_off, int nepoch, const double* d_t, const double* d_mobs,
+                              const double* d_sig, double* d_chi2, double* d_grad) {
+  if (!map) return MLC_ERR_ARG;
+  mlc_ctx* ctx = map->ctx;
+  return guarded(ctx, [&] {
+    Args a{map->precision, d_nodes, nnodes, map->Nx, map->Ny, P, B, node_off, node_cnt, node_rad,
+           band_scale, ncurves, d_params, d_prof, d_obs, nobs, d_obs_off, nepoch, d_t, d_mobs, d_sig,
+           d_chi2, d_grad};
+    validate_fused(a);
+    MLC_REQUIRE(node_shape, "lc_chi2_grad_convolve: null node shapes");
+    std::vector<ProfileParams> Ps(nnodes);
+    for (int k = 0; k < nnodes; ++k)
+      Ps[k] = checked_params(node_shape[k], node_rad[k], pix, 0.0, 1.0, map->Nx, map->Ny);
+    const size_t bytes = 2 * (map->precision == MLC_F64 ? sizeof(double) : sizeof(float)) *
+                         (size_t)map->Nx * map->Ny;
+    std::vector<void*> dst(nnodes);
+    for (int k = 0; k < nnodes; ++k) dst[k] = (char*)d_nodes + bytes * k;
+    // node k is convolved just before the first band launch that reads it,
+    // so that launch finds it in the last-level cache; nodes no band reads
+    // (and all nodes when there are no curves) follow at the end
+    std::vector<char> done(nnodes, 0);
+    NodeDriver drive = [&](const std::function<void(int)>& conv) {
+      std::function<void(int)> before = [&](int pb) {
+        for (int k = node_off[pb]; k < node_off[pb] + node_cnt[pb]; ++k)
+          if (!done[k]) {
+            conv(k);
+            done[k] = 1;
+          }
+      };
+      run_fused(ctx, a, &before);
+      for (int k = 0; k < nnodes; ++k)
+        if (!done[k]) conv(k);
+    };
+    if (map->precision == MLC_F64) fft_convolve_nodes<double>(map, Ps, dst.data(), &drive);
+    else fft_convolve_nodes<float>(map, Ps, dst.data(), &drive);
   });
 }
 

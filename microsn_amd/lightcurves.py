@@ -110,8 +110,13 @@ class Observations:
         self.nepoch = int(off[-1])
 
 
-def chi2_grad(ctx, nodes: NodeSet, obs: Observations, params, prof, obs_idx, plain=False):
-    """(chi2 [n], grad [n, 7]) FP64 device tensors for n curves."""
+def chi2_grad(ctx, nodes: NodeSet, obs: Observations, params, prof, obs_idx, plain=False, convolve=False):
+    """(chi2 [n], grad [n, 7]) FP64 device tensors for n curves.
+
+    convolve=True first recomputes the node maps from the spectrum's current
+    map, in the same call (mlc_lc_chi2_grad_convolve: each node right before
+    the band launch that reads it); the result equals nodes.compute()
+    followed by chi2_grad(...) bit for bit."""
     _dev(ctx, params, torch.float64, "params")
     _dev(ctx, prof, torch.int32, "prof")
     _dev(ctx, obs_idx, torch.int32, "obs")
@@ -119,6 +124,17 @@ def chi2_grad(ctx, nodes: NodeSet, obs: Observations, params, prof, obs_idx, pla
     assert params.shape == (n, NPARAM) and prof.shape == (n,) and obs_idx.shape == (n,)
     chi2 = torch.empty(n, dtype=torch.float64, device=ctx.device)
     grad = torch.empty((n, NPARAM), dtype=torch.float64, device=ctx.device)
+    if convolve:
+        assert not plain, "convolve=True runs the fused path"
+        if not hasattr(nodes, "_shp"):
+            nodes._shp = np.ascontiguousarray(nodes.node_shape, np.int32)
+        shp = nodes._shp
+        ctx.check(lib().mlc_lc_chi2_grad_convolve(
+            nodes.spectrum.h, shp.ctypes.data, nodes.pix, ptr(nodes.nodes), nodes.nodes.shape[0], nodes.P,
+            nodes.B, ptr(nodes.node_off), ptr(nodes.node_cnt), ptr(nodes.node_rad), ptr(nodes.band_scale), n,
+            ptr(params), ptr(prof), ptr(obs_idx), obs.nobs, ptr(obs.off), obs.nepoch, ptr(obs.t),
+            ptr(obs.mobs), ptr(obs.sig), ptr(chi2), ptr(grad)))
+        return chi2, grad
     f = lib().mlc_lc_chi2_grad_plain if plain else lib().mlc_lc_chi2_grad
     ctx.check(f(ctx.h, nodes.prec, ptr(nodes.nodes), nodes.nodes.shape[0], nodes.Nx, nodes.Ny, nodes.P,
                 nodes.B, ptr(nodes.node_off), ptr(nodes.node_cnt), ptr(nodes.node_rad),

@@ -222,3 +222,28 @@ def test_autograd_function(ctx):
     c2, g2 = lcm.chi2_grad(ctx, nodes, obs, P.detach(), pr, ob)
     assert torch.equal(c.detach(), c2)
     assert torch.allclose(P.grad, w[:, None] * g2, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_convolve_entry_matches_separate_calls(ctx, prec):
+    """mlc_lc_chi2_grad_convolve (node k convolved right before the band
+    launch that reads it) against mlc_map_convolve_nodes followed by
+    mlc_lc_chi2_grad: node maps, chi2 and gradient bit for bit; with no
+    curves every node map is still written."""
+    rng = np.random.default_rng(77)
+    _, _, nodes, obs, _ = _setup(ctx, prec, rng)
+    prm, prof, obsi = _params(rng, 500, 128, 160, nodes.P, 3)
+    prm[11, 0] = -5.0
+    args = (torch.tensor(prm, device="cuda"), torch.tensor(prof, device="cuda"), torch.tensor(obsi, device="cuda"))
+    nodes.compute()
+    ref = nodes.nodes.clone()
+    c0, g0 = lcm.chi2_grad(ctx, nodes, obs, *args)
+    nodes.nodes.fill_(float("nan"))
+    c1, g1 = lcm.chi2_grad(ctx, nodes, obs, *args, convolve=True)
+    assert torch.equal(nodes.nodes, ref)
+    assert not torch.isfinite(c0[11]) and torch.isfinite(c0).sum() > 400
+    assert torch.equal(c1, c0) and torch.equal(g1, g0)
+    nodes.nodes.fill_(float("nan"))
+    e = (args[0][:0].contiguous(), args[1][:0].contiguous(), args[2][:0].contiguous())
+    c2, g2 = lcm.chi2_grad(ctx, nodes, obs, *e, convolve=True)
+    assert c2.shape == (0,) and torch.equal(nodes.nodes, ref)
