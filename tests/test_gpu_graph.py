@@ -33,29 +33,33 @@ def _workload(ctx, N=512, n=4096, seed=77):
                 bins=api.mpd_bins(50, 1.0, 1e7))
 
 
-def _step(ctx, w):
+def _step(ctx, w, convolve=False):
     w["spec"].update(w["m"])
-    w["nodes"].compute()
-    c, g = lcm.chi2_grad(ctx, w["nodes"], w["obs"], w["P"], w["prof"], w["oi"])
+    if not convolve:
+        w["nodes"].compute()
+    # convolve=True: node maps and chi2 in one call
+    c, g = lcm.chi2_grad(ctx, w["nodes"], w["obs"], w["P"], w["prof"], w["oi"], convolve=convolve)
     perm, _, sk = api.sort_f64(ctx, c, sorted_keys=True)
     lo, npb, bc, be = api.bin_table_device(ctx, sk, w["bins"])
-    return c, g, perm, sk, lo, npb, bc, be
+    return c, g, perm, sk, lo, npb, bc, be, w["nodes"].nodes
 
 
-def test_captured_step_replays_match_eager(ctx):
+@pytest.mark.parametrize("convolve", [False, True])
+def test_captured_step_replays_match_eager(ctx, convolve):
     w = _workload(ctx)
-    eager = [t.clone() for t in _step(ctx, w)]
+    _step_c = lambda c, w: _step(c, w, convolve)
+    eager = [t.clone() for t in _step_c(ctx, w)]
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     ctx.set_stream(s)
     try:
         with torch.cuda.stream(s):
-            _step(ctx, w)  # the workspace, tables and caches settle on this stream
+            _step_c(ctx, w)  # the workspace, tables and caches settle on this stream
         s.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph, stream=s):
-            out = _step(ctx, w)
+            out = _step_c(ctx, w)
     finally:
         ctx.set_stream(torch.cuda.current_stream())
     for r in range(3):
@@ -66,6 +70,6 @@ def test_captured_step_replays_match_eager(ctx):
         for a, b in zip(out, eager):
             assert torch.equal(a, b), r
         # eager work between replays, on the library's usual stream
-        again = _step(ctx, w)
+        again = _step_c(ctx, w)
         torch.cuda.synchronize()
         assert torch.equal(again[0], eager[0])
