@@ -465,15 +465,57 @@ __global__ void k_upper_bound(const double* __restrict__ s, int64_t n, const dou
   upper[i] = (unsigned)lo;
 }
 
-// lower[i] = upper[i-1] (0 for i = 0), npb[i] = upper[i] - lower[i]
-// (setup_integral_GPU's adjacent_difference, gpu_functions.cu:389-397).
-__global__ void k_bin_table(const unsigned* __restrict__ upper, int nb, unsigned* __restrict__ lower,
-                            unsigned* __restrict__ npb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// First index in [lo, hi) whose key satisfies pred (pred is false, then
+// true along the range), or hi.  All 64 lanes of a wavefront call it
+// together: each round probes 64 evenly spaced keys at once and keeps the
+// gap between the last false and the first true probe, so a range of n keys
+// takes about log64(n) dependent reads instead of log2(n).
+template <class Pred>
+__device__ int64_t wave_partition_point(const double* __restrict__ s, int64_t lo, int64_t hi, Pred pred) {
+  const int l = threadIdx.x & 63;
+  while (hi - lo > 64) {
+    const int64_t step = (hi - lo + 63) / 64;
+    const int64_t p = lo + (int64_t)l * step;
+    const bool valid = p < hi;
+    const unsigned long long bt = __ballot(valid && pred(s[p]));
+    if (bt == 0) {
+      const int last = 63 - __builtin_clzll(__ballot(valid));
+      lo += (int64_t)last * step + 1;
+    } else {
+      const int f = __builtin_ctzll(bt);
+      if (f == 0) return lo;
+      hi = lo + (int64_t)f * step;
+      lo += (int64_t)(f - 1) * step + 1;
+    }
+  }
+  const int64_t p = lo + l;
+  const unsigned long long bt = __ballot(p < hi && pred(s[p]));
+  return bt ? lo + __builtin_ctzll(bt) : hi;
+}
+
+// The bin table of an ascending array (NaNs last) in one launch, one
+// wavefront per bin: upper[i] = #{non-NaN keys <= bins[i]}, lower[i] =
+// upper[i-1] (0 for i = 0), npb[i] = upper[i] - lower[i] -- the values
+// of setup_integral_GPU's upper_bound and adjacent_difference
+// (gpu_functions.cu:386-397); k_upper_bound gives the same upper[i].
+__global__ __launch_bounds__(64) void k_bin_table_waves(const double* __restrict__ s, int64_t n,
+                                                        const double* __restrict__ bins, int nb,
+                                                        unsigned* __restrict__ upper,
+                                                        unsigned* __restrict__ lower,
+                                                        unsigned* __restrict__ npb) {
+  const int i = blockIdx.x;
   if (i >= nb) return;
-  const unsigned lo = i ? upper[i - 1] : 0u;
-  lower[i] = lo;
-  npb[i] = upper[i] - lo;
+  const int64_t nvalid = wave_partition_point(s, 0, n, [](double v) { return v != v; });
+  auto ub = [&](double b) {
+    return (unsigned)wave_partition_point(s, 0, nvalid, [b](double v) { return b < v; });
+  };
+  const unsigned up = ub(bins[i]);
+  const unsigned lo = i ? ub(bins[i - 1]) : 0u;
+  if (threadIdx.x == 0) {
+    upper[i] = up;
+    lower[i] = lo;
+    npb[i] = up - lo;
+  }
 }
 
 template <int R>
@@ -557,10 +599,8 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
 // host round trip.
 void bin_edges_device(mlc_ctx* ctx, const double* sorted, int64_t n, const double* d_bins, int Nbins,
                       unsigned* d_upper, unsigned* d_lower, unsigned* d_npb) {
-  MLC_LAUNCH(ctx, "bin_upper_bound", k_upper_bound, dim3(cdiv(Nbins, 64)), dim3(64), 0, sorted, n,
-             d_bins, Nbins, d_upper);
-  MLC_LAUNCH(ctx, "bin_table", k_bin_table, dim3(cdiv(Nbins, 256)), dim3(256), 0, d_upper, Nbins, d_lower,
-             d_npb);
+  MLC_LAUNCH(ctx, "bin_table", k_bin_table_waves, dim3(Nbins), dim3(64), 0, sorted, n, d_bins, Nbins,
+             d_upper, d_lower, d_npb);
 }
 
 void bin_edges(mlc_ctx* ctx, const double* sorted, int64_t n, const double* bins, int Nbins,
