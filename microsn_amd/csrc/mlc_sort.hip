@@ -269,8 +269,10 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __rest
                                                           unsigned* __restrict__ pout) {
   __shared__ uint64_t sk[kSeg];
   __shared__ unsigned sp[kSeg];
-  __shared__ unsigned wave_cnt[kWaves][256];
-  __shared__ unsigned run[256];    // keys of each digit ranked so far in this pass
+  // cnt[r][w][d]: keys of digit d in round r of wave w; then, in place, the
+  // number of keys of digit d before that (round, wave) in segment order
+  __shared__ __align__(16) unsigned cnt[kSegR][kWaves][256];
+  __shared__ unsigned wsum[kWaves];
   __shared__ unsigned start[256];  // first slot of each digit (exclusive scan)
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t base = (int64_t)blockIdx.x * kSeg;
@@ -280,12 +282,13 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __rest
     sp[q] = (unsigned)(base + q);
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  static_assert(kSegThreads == 256, "one thread per digit");
   for (int pass = 0; pass < 8; ++pass) {
     if (pass_trivial(hist + pass * 256, n)) continue;  // block-uniform; also a barrier
-    run[t] = 0;
-    for (int i = t; i < kWaves * 256; i += kSegThreads) (&wave_cnt[0][0])[i] = 0;
+    for (int i = t; i < kSegR * kWaves * 64; i += kSegThreads)
+      reinterpret_cast<uint4*>(&cnt[0][0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
     uint64_t key[kSegR];
-    unsigned pay[kSegR], rank[kSegR], dig[kSegR];
+    unsigned pay[kSegR], dig[kSegR], before[kSegR];
 #pragma unroll
     for (int r = 0; r < kSegR; ++r) {
       const int q = r * kSegThreads + t;
@@ -294,10 +297,11 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __rest
       dig[r] = q < valid ? digit_of(key[r], pass) : 256u;  // 256: padding, not ranked
     }
     __syncthreads();
+    // rank within the wave: lanes of this wave holding the same digit
+    // (padding lanes match nobody); the first of them records the count
 #pragma unroll
     for (int r = 0; r < kSegR; ++r) {
       const unsigned d = dig[r];
-      // lanes of this wave holding the same digit (padding lanes match nobody)
       uint64_t peers = __ballot(d < 256u);
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
@@ -305,44 +309,38 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __rest
         peers &= ((d >> b) & 1u) ? m : ~m;
       }
       if (d >= 256u) peers = 0;
-      const unsigned before = (unsigned)__popcll(peers & lt);
-      if (d < 256u && before == 0) wave_cnt[w][d] = (unsigned)__popcll(peers);
-      __syncthreads();
-      if (d < 256u) {
-        unsigned p = run[d] + before;
-        for (int ww = 0; ww < w; ++ww) p += wave_cnt[ww][d];
-        rank[r] = p;
-      }
-      __syncthreads();
-      {
-        unsigned add = 0;
-        for (int ww = 0; ww < kWaves; ++ww) {
-          add += wave_cnt[ww][t];
-          wave_cnt[ww][t] = 0;
-        }
-        run[t] += add;
-      }
-      __syncthreads();
+      before[r] = (unsigned)__popcll(peers & lt);
+      if (d < 256u && before[r] == 0) cnt[r][w][d] = (unsigned)__popcll(peers);
     }
-    {  // exclusive scan of the digit counts (run[]) -> start[]
-      const unsigned v = run[t];
-      start[t] = v;
-      __syncthreads();
-      for (int off = 1; off < 256; off <<= 1) {
-        const unsigned a = t >= off ? start[t - off] : 0u;
-        __syncthreads();
-        start[t] += a;
-        __syncthreads();
+    __syncthreads();
+    // thread t owns digit t: exclusive prefix over (round, wave), which is
+    // segment order (q = round * 256 + wave * 64 + lane), then the digit's
+    // start from a scan of the totals over the digits
+    unsigned tot = 0;
+#pragma unroll
+    for (int r = 0; r < kSegR; ++r)
+#pragma unroll
+      for (int ww = 0; ww < kWaves; ++ww) {
+        const unsigned v = cnt[r][ww][t];
+        cnt[r][ww][t] = tot;
+        tot += v;
       }
-      const unsigned incl = start[t];
-      __syncthreads();
-      start[t] = incl - v;
-      __syncthreads();
+    unsigned incl = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
     }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    unsigned pre = 0;
+    for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+    start[t] = pre + incl - tot;
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSegR; ++r) {
       if (dig[r] < 256u) {
-        const unsigned q = start[dig[r]] + rank[r];
+        const unsigned q = start[dig[r]] + cnt[r][w][dig[r]] + before[r];
         sk[q] = key[r];
         sp[q] = pay[r];
       }
