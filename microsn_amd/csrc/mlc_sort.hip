@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
 
 // ---- sorts of up to kSmallSort keys: segments sorted in LDS, then merged.
 constexpr int kSegThreads = 256, kSegR = 8, kSeg = kSegThreads * kSegR;  // keys per segment
-constexpr int kMergeThreads = 256, kMergeItems = 8;  // outputs per thread of a merge round
+constexpr int kMergeThreads = 256, kMergeItems = 4;  // outputs per thread of a merge round
 constexpr int kMergeTile = kMergeThreads * kMergeItems;  // outputs per workgroup
 static_assert((2 * kSeg) % kMergeTile == 0, "a merge workgroup stays inside one pair of runs");
 
@@ -382,7 +382,8 @@ __device__ __forceinline__ int64_t merge_split_wave(const uint64_t* __restrict__
 // outputs, all inside one pair of runs (2L is a multiple of kMergeTile):
 // two wavefronts find where its first and last outputs split the pair, the
 // workgroup copies those stretches of both runs to LDS, and each thread
-// merges kMergeItems outputs there after its own merge-path search.
+// merges kMergeItems outputs there after its own merge-path search; the
+// merged tile goes back through LDS so that its stores are contiguous.
 // Stable: on equal keys the left run's element comes first.
 __global__ __launch_bounds__(kMergeThreads) void k_merge(const uint64_t* __restrict__ kin,
                                                          const unsigned* __restrict__ pin,
@@ -411,20 +412,39 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(const uint64_t* __restr
   }
   __syncthreads();
   const int e0 = t * kMergeItems, ne = (int)(d1 - d0);
-  if (e0 >= ne) return;
-  int lo = max(0, e0 - nb), hi = min(e0, na);
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (sk[mid] <= sk[na + e0 - 1 - mid]) lo = mid + 1;
-    else hi = mid;
+  uint64_t ok[kMergeItems];
+  unsigned op[kMergeItems];
+  if (e0 < ne) {
+    int lo = max(0, e0 - nb), hi = min(e0, na);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (sk[mid] <= sk[na + e0 - 1 - mid]) lo = mid + 1;
+      else hi = mid;
+    }
+    int i = lo, j = na + (e0 - lo);
+    const int jend = na + nb;
+#pragma unroll
+    for (int e = 0; e < kMergeItems; ++e) {
+      if (e0 + e < ne) {
+        const bool take_a = j >= jend || (i < na && sk[i] <= sk[j]);
+        const int src = take_a ? i++ : j++;
+        ok[e] = sk[src];
+        op[e] = sp[src];
+      }
+    }
   }
-  int i = lo, j = na + (e0 - lo);
-  const int jend = na + nb;
-  for (int e = 0; e < kMergeItems && e0 + e < ne; ++e) {
-    const bool take_a = j >= jend || (i < na && sk[i] <= sk[j]);
-    const int src = take_a ? i++ : j++;
-    kout[o0 + e0 + e] = sk[src];
-    pout[o0 + e0 + e] = sp[src];
+  // through LDS again, so that the stores to global memory are contiguous
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kMergeItems; ++e)
+    if (e0 + e < ne) {
+      sk[e0 + e] = ok[e];
+      sp[e0 + e] = op[e];
+    }
+  __syncthreads();
+  for (int q = t; q < ne; q += kMergeThreads) {
+    kout[o0 + q] = sk[q];
+    pout[o0 + q] = sp[q];
   }
 }
 
