@@ -259,12 +259,12 @@ constexpr int kMergeThreads = 256, kMergeItems = 4;  // outputs per thread of a 
 constexpr int kMergeTile = kMergeThreads * kMergeItems;  // outputs per workgroup
 static_assert((2 * kSeg) % kMergeTile == 0, "a merge workgroup stays inside one pair of runs");
 
-// Stable sort of one segment of kSeg transformed keys, with their original
-// indices as payload, entirely in LDS: eight 8-bit LSD passes, each ranking
-// keys stably as k_scatter does (segment order = (round, thread)); passes
-// whose digit is the same for every key of the whole input are skipped.
-__global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __restrict__ kin, int64_t n,
-                                                          const unsigned* __restrict__ hist,
+// Stable sort of one segment of kSeg keys (the doubles, turned into ordered
+// integer keys as k_transform_hist does), with their original indices as
+// payload, entirely in LDS: eight 8-bit LSD passes, each ranking keys stably
+// as k_scatter does (segment order = (round, thread)); a pass whose digit
+// is the same for every key of the segment is the identity and is skipped.
+__global__ __launch_bounds__(kSegThreads) void k_seg_sort(const double* __restrict__ in, int64_t n,
                                                           uint64_t* __restrict__ kout,
                                                           unsigned* __restrict__ pout) {
   __shared__ uint64_t sk[kSeg];
@@ -274,17 +274,42 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const uint64_t* __rest
   __shared__ __align__(16) unsigned cnt[kSegR][kWaves][256];
   __shared__ unsigned wsum[kWaves];
   __shared__ unsigned start[256];  // first slot of each digit (exclusive scan)
+  __shared__ uint64_t red[2][kWaves];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t base = (int64_t)blockIdx.x * kSeg;
   const int valid = (int)min<int64_t>(kSeg, n - base);
+  uint64_t kand = ~0ull, kor = 0ull;  // bits shared by all keys of the segment
   for (int q = t; q < kSeg; q += kSegThreads) {
-    sk[q] = q < valid ? kin[base + q] : ~0ull;
+    const uint64_t k = q < valid ? key_of(in[base + q]) : ~0ull;
+    sk[q] = k;
     sp[q] = (unsigned)(base + q);
+    if (q < valid) {
+      kand &= k;
+      kor |= k;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    kand &= __shfl_xor(kand, off, 64);
+    kor |= __shfl_xor(kor, off, 64);
+  }
+  if (lane == 0) {
+    red[0][w] = kand;
+    red[1][w] = kor;
+  }
+  __syncthreads();
+  uint64_t varying = 0;  // bits that differ between two keys of the segment
+  {
+    uint64_t a = ~0ull, o = 0ull;
+    for (int ww = 0; ww < kWaves; ++ww) {
+      a &= red[0][ww];
+      o |= red[1][ww];
+    }
+    varying = a ^ o;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   static_assert(kSegThreads == 256, "one thread per digit");
   for (int pass = 0; pass < 8; ++pass) {
-    if (pass_trivial(hist + pass * 256, n)) continue;  // block-uniform; also a barrier
+    if (((varying >> (8 * pass)) & 255u) == 0) continue;  // block-uniform
     for (int i = t; i < kSegR * kWaves * 64; i += kSegThreads)
       reinterpret_cast<uint4*>(&cnt[0][0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
     uint64_t key[kSegR];
@@ -583,19 +608,13 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   ws.commit();
   uint64_t *k0 = ws.get<uint64_t>(ik0), *k1 = ws.get<uint64_t>(ik1);
   unsigned *pbuf = ws.get<unsigned>(ip1), *hist = ws.get<unsigned>(ih), *cnt = ws.get<unsigned>(ic);
-  // a fill kernel, not a memset node, so that a captured graph replays it
-  MLC_LAUNCH(ctx, "sort_zero_hist", k_zero_u32, dim3(8), dim3(256), 0, hist, 8 * 256);
-  const int g = (int)std::min<int64_t>(cdiv(n, kThreads * 4), fill_grid(ctx, 1));
-  MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
-             hist);
   if (small) {
     // segments, then merge rounds; the segment sort writes the payload where
     // an even number of rounds leaves it in perm
     int rounds = 0;
     for (int64_t L = kSeg; L < n; L *= 2) ++rounds;
     unsigned* p0 = rounds % 2 == 0 ? perm : pbuf;
-    MLC_LAUNCH(ctx, "sort_segments", k_seg_sort, dim3(cdiv(n, kSeg)), dim3(kSegThreads), 0, k0, n, hist, k1,
-               p0);
+    MLC_LAUNCH(ctx, "sort_segments", k_seg_sort, dim3(cdiv(n, kSeg)), dim3(kSegThreads), 0, keys, n, k1, p0);
     uint64_t *kin = k1, *kout = k0;
     unsigned *pin = p0, *pout = p0 == perm ? pbuf : perm;
     for (int64_t L = kSeg; L < n; L *= 2) {
@@ -605,6 +624,11 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
       std::swap(pin, pout);
     }
   } else {
+    // a fill kernel, not a memset node, so that a captured graph replays it
+    MLC_LAUNCH(ctx, "sort_zero_hist", k_zero_u32, dim3(8), dim3(256), 0, hist, 8 * 256);
+    const int g = (int)std::min<int64_t>(cdiv(n, kThreads * 4), fill_grid(ctx, 1));
+    MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
+               hist);
     sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
   }
   if (rank || sorted)
