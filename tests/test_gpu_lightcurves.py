@@ -247,3 +247,40 @@ def test_convolve_entry_matches_separate_calls(ctx, prec):
     e = (args[0][:0].contiguous(), args[1][:0].contiguous(), args[2][:0].contiguous())
     c2, g2 = lcm.chi2_grad(ctx, nodes, obs, *e, convolve=True)
     assert c2.shape == (0,) and torch.equal(nodes.nodes, ref)
+
+
+def test_convolve_entry_rejects_bad_tables(ctx):
+    """Bad node shapes or node tables are refused before any launch: the
+    status is MLC_ERR_ARG and the node maps keep their contents."""
+    import ctypes as C
+    from microsn_amd._lib import lib, ptr
+
+    rng = np.random.default_rng(78)
+    _, _, nodes, obs, _ = _setup(ctx, "f32", rng)
+    prm, prof, obsi = _params(rng, 50, 128, 160, nodes.P, 3)
+    P_, pr, oi = (torch.tensor(prm, device="cuda"), torch.tensor(prof, device="cuda"),
+                  torch.tensor(obsi, device="cuda"))
+    chi2 = torch.empty(50, dtype=torch.float64, device="cuda")
+    grad = torch.empty((50, 7), dtype=torch.float64, device="cuda")
+    nodes.nodes.fill_(7.0)
+    torch.cuda.synchronize()
+
+    def call(shp, off):
+        return lib().mlc_lc_chi2_grad_convolve(
+            nodes.spectrum.h, shp.ctypes.data, nodes.pix, ptr(nodes.nodes), nodes.nodes.shape[0], nodes.P,
+            nodes.B, off.ctypes.data, ptr(nodes.node_cnt), ptr(nodes.node_rad), ptr(nodes.band_scale), 50,
+            ptr(P_), ptr(pr), ptr(oi), obs.nobs, ptr(obs.off), obs.nepoch, ptr(obs.t), ptr(obs.mobs),
+            ptr(obs.sig), ptr(chi2), ptr(grad))
+
+    good_shp = np.ascontiguousarray(nodes.node_shape, np.int32)
+    bad_shp = good_shp.copy()
+    bad_shp[-1] = 99
+    bad_off = nodes.node_off.copy()
+    bad_off[-1] = nodes.nodes.shape[0]  # past the last node
+    assert call(bad_shp, nodes.node_off) == 1
+    assert call(good_shp, bad_off) == 1
+    torch.cuda.synchronize()
+    assert bool((nodes.nodes == 7.0).all())
+    assert call(good_shp, nodes.node_off) == 0
+    torch.cuda.synchronize()
+    assert not bool((nodes.nodes == 7.0).all())
