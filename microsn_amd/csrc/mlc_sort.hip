@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
 
 // ---- sorts of up to kSmallSort keys: segments sorted in LDS, then merged.
 constexpr int kSegThreads = 256, kSegR = 8, kSeg = kSegThreads * kSegR;  // keys per segment
-constexpr int kMergeThreads = 256, kMergeItems = 4;  // outputs per thread of a merge round
+constexpr int kMergeThreads = 256, kMergeItems = 2;  // outputs per thread of a merge round
 constexpr int kMergeTile = kMergeThreads * kMergeItems;  // outputs per workgroup
 static_assert((2 * kSeg) % kMergeTile == 0, "a merge workgroup stays inside one pair of runs");
 
