@@ -184,12 +184,16 @@ def capture_step(ctx, ws):
             out = run_step(ctx, ws)
     finally:
         ctx.set_stream(torch.cuda.current_stream())
-    return g, out
+    return g, out, ctx.generation()
 
 
-def time_graph(g, steps, warmup):
+def time_graph(ctx, g, gen, steps, warmup):
+    """Replays of a graph captured at context generation gen (the library
+    has reallocated or rewritten nothing it refers to since)."""
     import torch
 
+    if ctx.generation() != gen:
+        raise RuntimeError("the captured step refers to memory the library has since replaced")
     for _ in range(warmup):
         g.replay()
     torch.cuda.synchronize()
@@ -198,6 +202,18 @@ def time_graph(g, steps, warmup):
         g.replay()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(m, params, prof, times, mobs, sig, bins):
@@ -233,6 +249,7 @@ def cpu_baseline(m, params, prof, times, mobs, sig, bins):
     t_step = time.perf_counter() - t0
     ncurves = len(prof)
     return {"value": ncurves / t_step, "unit": "light curves/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": (f"one whole step on the box's host with the FP64 oracle: 9 node maps as 18 "
                        f"4096^2 FFT convolutions ({t_conv:.2f} s; each transforms the map again, "
                        f"54 2-D FFTs where 37 would do), chi2 + adjoint of {ncurves} curves x 300 "
@@ -316,8 +333,8 @@ def main():
     dt32 = dt_eager
     if not args.eager:
         # the same step as one graph replay; its outputs must equal the eager step's
-        g32, gout = capture_step(ctx, ws32)
-        dt32 = time_graph(g32, args.steps, args.warmup)
+        g32, gout, gen = capture_step(ctx, ws32)
+        dt32 = time_graph(ctx, g32, gen, args.steps, args.warmup)
         same = all(torch.equal(a, b) for a, b in zip(gout, out32))
         if not same:
             print("graph replay differs from the eager step: timing the eager step", file=sys.stderr)
@@ -336,6 +353,9 @@ def main():
         ws64 = setup(ctx, "f64", m_dev, params, prof, times, mobs, sig)
         k64 = max(2, args.steps // 5)
         dt64, out64, _ = time_steps(ctx, ws64, k64, 1, plain=True, timing=False)
+        # one more step with an event pair around every launch: the plain
+        # path's per-kernel breakdown (its own roofline beside the FP32 one)
+        _, _, kern64 = time_steps(ctx, ws64, 1, 0, plain=True, timing=True)
         chi64, grad64, lo64, npb64, be64 = out64
         lo32, npb32, lo64, npb64 = (x.cpu().numpy() for x in (lo32, npb32, lo64, npb64))
         fin = torch.isfinite(chi64)
@@ -350,7 +370,14 @@ def main():
                "lightcurve_max_err_mag": 2.5 / np.log(10) * map_err,
                "bins_identical_to_fp64": bool(np.array_equal(lo32, lo64) and np.array_equal(npb32, npb64)),
                "curves_changing_bin": int(np.abs(npb32.astype(np.int64) - npb64.astype(np.int64)).sum() // 2)}
-        plain = {"value": ncurves / dt64, "ms_per_step": dt64 * 1e3, "steps": k64, "dtype": "f64"}
+        plain = {"value": ncurves / dt64, "ms_per_step": dt64 * 1e3, "steps": k64, "dtype": "f64",
+                 "kernels_ms_per_step": {k: v[0] for k, v in sorted(kern64.items())},
+                 "roofline_by_kernel": {
+                     k: {"achieved": algorithmic_bytes(k, 8, ncurves) / (v[0] / v[1] / 1e3) / 1e9,
+                         "unit": "GB/s",
+                         "frac": algorithmic_bytes(k, 8, ncurves) / (v[0] / v[1] / 1e3) / HBM_PEAK,
+                         "avg_launch_ms": v[0] / v[1]}
+                     for k, v in kern64.items() if algorithmic_bytes(k, 8, ncurves)}}
         del ws64
         torch.cuda.empty_cache()
 

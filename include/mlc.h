@@ -65,6 +65,17 @@ const char* mlc_last_error(const mlc_ctx* ctx);
  * device tables: a later call that needs a larger workspace, or uploads
  * different bin edges or node jobs, invalidates it. */
 int mlc_ctx_set_stream(mlc_ctx* ctx, void* stream);
+/* The context's generation: a counter bumped whenever the library
+ * reallocates device memory that queued or captured work may point at (the
+ * workspace, a map's node tables) or rewrites one of its device tables in
+ * place.  A graph captured at generation g may be replayed while
+ * mlc_ctx_generation still returns g.  No reference counterpart (the
+ * reference has no graph capture). */
+int mlc_ctx_generation(const mlc_ctx* ctx, uint64_t* out);
+/* Grows the context's workspace to at least `bytes` now (one allocation), so
+ * that later calls up to that size neither reallocate it nor bump the
+ * generation; used before a capture. */
+int mlc_ctx_reserve(mlc_ctx* ctx, size_t bytes);
 int mlc_ctx_sync(mlc_ctx* ctx);
 int mlc_malloc(mlc_ctx* ctx, size_t bytes, void** d_ptr);
 int mlc_free(mlc_ctx* ctx, void* d_ptr);

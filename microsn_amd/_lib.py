@@ -26,6 +26,8 @@ SIGNATURES = {
     "mlc_ctx_destroy": (None, [_p]),
     "mlc_last_error": (C.c_char_p, [_p]),
     "mlc_ctx_set_stream": (_i, [_p, _p]),
+    "mlc_ctx_generation": (_i, [_p, C.POINTER(C.c_uint64)]),
+    "mlc_ctx_reserve": (_i, [_p, _sz]),
     "mlc_ctx_sync": (_i, [_p]),
     "mlc_malloc": (_i, [_p, _sz, _pp]),
     "mlc_free": (_i, [_p, _p]),

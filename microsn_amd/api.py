@@ -61,6 +61,18 @@ class Context:
     def sync(self):
         self.check(lib().mlc_ctx_sync(self.h))
 
+    def generation(self):
+        """Counter bumped whenever the library reallocates memory or rewrites
+        a device table that a captured graph may refer to: a graph captured
+        at generation g replays correctly while this still returns g."""
+        g = C.c_uint64()
+        self.check(lib().mlc_ctx_generation(self.h, C.byref(g)))
+        return g.value
+
+    def reserve(self, nbytes):
+        """Grow the workspace to at least nbytes now (before a capture)."""
+        self.check(lib().mlc_ctx_reserve(self.h, int(nbytes)))
+
     def close(self):
         if getattr(self, "h", None):
             for m in list(getattr(self, "_maps", ())):
@@ -198,6 +210,11 @@ def parity_round(x):
 
 
 def fft_r2c(ctx, x):
+    """Unnormalised real 2-D FFT of a float32/float64 Ny x Nx device tensor."""
+    if x.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"fft_r2c: input must be float32 or float64, not {x.dtype}")
+    if x.dim() != 2:
+        raise ValueError(f"fft_r2c: input must be 2-D, not {tuple(x.shape)}")
     prec = F64 if x.dtype == torch.float64 else F32
     _dev(ctx, x, x.dtype, "input")
     Ny, Nx = x.shape
@@ -208,6 +225,11 @@ def fft_r2c(ctx, x):
 
 
 def fft_c2r(ctx, X, Nx):
+    """Unnormalised inverse of fft_r2c: complex64/complex128 Ny x (Nx/2+1) in."""
+    if X.dtype not in (torch.complex64, torch.complex128):
+        raise TypeError(f"fft_c2r: input must be complex64 or complex128, not {X.dtype}")
+    if X.dim() != 2 or X.shape[1] != Nx // 2 + 1:
+        raise ValueError(f"fft_c2r: input must be Ny x {Nx // 2 + 1}, not {tuple(X.shape)}")
     prec = F64 if X.dtype == torch.complex128 else F32
     _dev(ctx, X, X.dtype, "input")
     Ny = X.shape[0]

@@ -49,6 +49,7 @@ const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes) 
     t.cap = std::max<size_t>(bytes, 256);
   }
   h2d(ctx, t.dev, host, bytes);
+  ++ctx->generation;  // a graph captured before this call would read the new table
   t.host.assign(static_cast<const char*>(host), static_cast<const char*>(host) + bytes);
   return t.dev;
 }
@@ -160,6 +161,20 @@ const char* mlc_last_error(const mlc_ctx* ctx) { return ctx ? ctx->err.c_str() :
 
 int mlc_ctx_set_stream(mlc_ctx* ctx, void* stream) {
   return guarded(ctx, [&] { ctx->stream = (hipStream_t)stream; });
+}
+
+int mlc_ctx_generation(const mlc_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return MLC_ERR_ARG;
+  *out = ctx->generation;
+  return MLC_OK;
+}
+
+int mlc_ctx_reserve(mlc_ctx* ctx, size_t bytes) {
+  return guarded(ctx, [&] {
+    Workspace ws(ctx);
+    ws.add(bytes);
+    ws.commit();
+  });
 }
 
 int mlc_ctx_sync(mlc_ctx* ctx) {

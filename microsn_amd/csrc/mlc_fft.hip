@@ -94,6 +94,8 @@ int mlc_map_create(mlc_ctx* ctx, int precision, const double* d_map, int Nx, int
     hip_check(hipMalloc(&m->g0, spec), "hipMalloc(column scratch)");
     hip_check(hipMalloc(&m->g1, spec), "hipMalloc(column scratch)");
     hip_check(hipMalloc((void**)&m->mean, sizeof(double)), "hipMalloc(mean)");
+    // the mean stays 0 unless subtract_mean: set once here, never by an update
+    hip_check(hipMemsetAsync(m->mean, 0, sizeof(double), ctx->stream), "hipMemsetAsync(mean)");
     if (precision == MLC_F64) fft_map_transform<double>(m, d_map);
     else fft_map_transform<float>(m, d_map);
     *out = m;

@@ -1278,8 +1278,9 @@ void fft_map_transform(mlc_map* m, const double* d_map) {
     MLC_LAUNCH(ctx, "map_dc", k_row_dc_fix<T>, dim3(1), dim3(1024), 0, rowsum, m->Nx, m->Ny,
                (cpx<T>*)m->rows, m->ldr, m->mean);
   } else {
-    hip_check(hipMemsetAsync(m->mean, 0, sizeof(double), ctx->stream), "hipMemsetAsync(mean)");
-    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, m->mean, nullptr, m->ldr);
+    // m->mean was zeroed once when the map was created and stays zero, so an
+    // update issues kernels only (no memset node under graph capture)
+    row_r2c<T, double>(ctx, d_map, (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, nullptr, m->ldr);
   }
   m->full_ready = false;
 }

@@ -50,6 +50,12 @@ struct mlc_ctx {
   void* ws = nullptr;
   size_t ws_bytes = 0;
 
+  // bumped whenever device memory that a captured graph may point at is
+  // reallocated (workspace, a map's node tables) or a device table is
+  // rewritten in place (const_table); a graph captured at generation g is
+  // valid while the generation is still g
+  uint64_t generation = 0;
+
   // per-kernel timing
   bool timing = false;
   struct Pending {
@@ -121,6 +127,7 @@ struct Workspace {
       size_t sz = used + (used >> 3) + 4096;
       hip_check(hipMalloc(buf, sz), "hipMalloc(workspace)");
       *cap = sz;
+      ++ctx->generation;
     }
   }
   template <class T>

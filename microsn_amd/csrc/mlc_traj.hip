@@ -498,14 +498,17 @@ __global__ __launch_bounds__(256) void k_lc_band(
         u[2] = c2.x + dr * c2.y; v[2] = c2.y;
         u[3] = c3.x + dr * c3.y; v[3] = c3.y;
       } else {
-        // node interval k, k+1; Taylor about the end node outside [rlo, rhi]
+        // node interval k, k+1; Taylor about the end node outside [rlo, rhi].
+        // A NaN radius counts as below the first node (its Taylor step gives
+        // a NaN mu, which flags the curve), and a Taylor step reads node ka
+        // twice, so kb never passes the last node, also when nn == 1.
         int k = 0;
         while (k < nn - 2 && srad[k + 1] <= r) ++k;
-        const bool lo = r <= rlo, hi = r >= rhi;
-        const int ka = hi ? nn - 1 : k, kb = hi ? nn - 1 : k + 1;
+        const bool lo = !(r > rlo), hi = !lo && r >= rhi;
+        const int ka = hi ? nn - 1 : k, kb = (lo || hi) ? ka : k + 1;
         const double ra = srad[ka], h = srad[kb] - ra;
         T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
-        if (lo || hi) {
+        if (ka == kb) {
           const T dr = (T)(r - ra);
           wa = 1; wda = dr; wb = 0; wdb = 0;
           za = 0; zda = 1; zb = 0; zdb = 0;

@@ -12,10 +12,13 @@
  * brute force and central finite differences in tests/.
  *
  * Parity unpinned: the d/drhalf of the uniform disc (a one-pixel edge ramp,
- * a model convention with no reference counterpart), and the values of the
- * thermal_hole and wavy profiles beyond their normalisation, symmetry,
- * zero regions and amplitude invariance (their d/drhalf is pinned by
- * central differences like the other shapes).
+ * a model convention with no reference counterpart).  Every other function
+ * is pinned in tests/ (test_oracle.py, test_oracle_pins.py, test_host.py,
+ * test_fft_core.py): profile values of all six shapes by numpy restatements
+ * of profile.cpp and the thin disc's half-light radius, the multi-band chi2
+ * bit for bit by a numpy float32 restatement of chi2_functions.cuh in both
+ * index conventions, and the trajectory chi2 and its gradient by closed
+ * forms on affine maps.
  */
 #ifndef MLC_ORACLE_H
 #define MLC_ORACLE_H

@@ -12,12 +12,14 @@ from microsn_amd import lightcurves as lcm
 pytestmark = pytest.mark.gpu
 
 
-def _workload(ctx, N=512, n=4096, seed=77):
+def _workload(ctx, N=512, n=4096, seed=77, prec="f32"):
     rng = np.random.default_rng(seed)
     g = torch.randn((N, N), dtype=torch.float64, device="cuda",
                     generator=torch.Generator(device="cuda").manual_seed(seed))
     m = torch.exp(0.5 * g) * 5.0
-    spec = api.MapSpectrum(ctx, m, "f32")
+    # FP64 maps keep the mean in the spectrum (subtract_mean off by default):
+    # their update must issue kernels only, like the FP32 path
+    spec = api.MapSpectrum(ctx, m, prec)
     shapes, bs = ("gaussian", "uniform"), (1.0, 2.0)
     nodes = lcm.NodeSet(ctx, spec, shapes, [[[2.0], [4.0]], [[2.0], [4.0]]], bs, compute=False)
     times = [np.sort(rng.uniform(0, 50, 30)) for _ in bs]
@@ -44,9 +46,9 @@ def _step(ctx, w, convolve=False):
     return c, g, perm, sk, lo, npb, bc, be, w["nodes"].nodes
 
 
-@pytest.mark.parametrize("convolve", [False, True])
-def test_captured_step_replays_match_eager(ctx, convolve):
-    w = _workload(ctx)
+@pytest.mark.parametrize("convolve,prec", [(False, "f32"), (True, "f32"), (True, "f64")])
+def test_captured_step_replays_match_eager(ctx, convolve, prec):
+    w = _workload(ctx, prec=prec)
     _step_c = lambda c, w: _step(c, w, convolve)
     eager = [t.clone() for t in _step_c(ctx, w)]
     torch.cuda.synchronize()
@@ -58,11 +60,15 @@ def test_captured_step_replays_match_eager(ctx, convolve):
             _step_c(ctx, w)  # the workspace, tables and caches settle on this stream
         s.synchronize()
         graph = torch.cuda.CUDAGraph()
+        gen = ctx.generation()
         with torch.cuda.graph(graph, stream=s):
             out = _step_c(ctx, w)
+        # the capture itself reallocated nothing and rewrote no table
+        assert ctx.generation() == gen
     finally:
         ctx.set_stream(torch.cuda.current_stream())
     for r in range(3):
+        assert ctx.generation() == gen, "a call since the capture invalidated the graph"
         for t in out:
             t.zero_()
         graph.replay()
