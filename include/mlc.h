@@ -315,6 +315,26 @@ int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix,
                               const int* d_obs_off, int nepoch, const double* d_t,
                               const double* d_mobs, const double* d_sig,
                               double* d_chi2, double* d_grad);
+/* mlc_lc_chi2_grad without any device allocation: the same kernels and the
+ * same results bit for bit, but the call's scratch (curve records, partition
+ * counts, per-curve accumulators, epoch records: about 136 bytes per curve
+ * plus 32 per epoch) comes from a static device array of 64 MiB per GPU
+ * (held by each GPU the library runs on until the process ends), not from
+ * the context's growable workspace.  A call whose scratch would exceed
+ * 64 MiB fails with MLC_ERR_ARG before any launch; nothing calls hipMalloc
+ * or hipFree.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two contexts, two host threads) race and
+ * must be serialised by the caller.  No counterpart in the reference; it is
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). */
+int mlc_lc_chi2_grad_noalloc(mlc_ctx* ctx, int precision, const void* d_nodes,
+                             int nnodes, int Nx, int Ny, int P, int B,
+                             const int* node_off, const int* node_cnt,
+                             const double* node_rad, const double* band_scale,
+                             int ncurves, const double* d_params, const int* d_prof,
+                             const int* d_obs, int nobs, const int* d_obs_off,
+                             int nepoch, const double* d_t, const double* d_mobs,
+                             const double* d_sig, double* d_chi2, double* d_grad);
 /* Same result, structured like the reference's plain pipeline: one launch
  * writes per-sample (mu, dmu/dx, dmu/dy, dmu/drad) to device memory, the
  * stream is synchronised, a second launch reduces chi2 and gradient.  The

@@ -68,6 +68,8 @@ SIGNATURES = {
                               _i, _p, _i, _p, _p, _p, _p, _p]),
     "mlc_lc_chi2_grad_convolve": (_i, [_p, _p, _d, _p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p, _i,
                                        _p, _i, _p, _p, _p, _p, _p]),
+    "mlc_lc_chi2_grad_noalloc": (_i, [_p, _i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p,
+                                      _p, _i, _p, _i, _p, _p, _p, _p, _p]),
     "mlc_lc_chi2_grad_plain": (_i, [_p, _i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p,
                                     _p, _i, _p, _i, _p, _p, _p, _p, _p]),
     # mlc_host.h

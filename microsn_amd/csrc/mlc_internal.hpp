@@ -107,8 +107,20 @@ struct Workspace {
   size_t* cap;
   size_t used = 0;
   std::vector<std::pair<size_t, size_t>> slices;  // offset, bytes
+  // fixed mode: a caller-provided buffer that is never reallocated
+  bool fixed = false;
+  void* fixed_buf = nullptr;
+  size_t fixed_cap = 0;
+  const char* fixed_what = nullptr;
   explicit Workspace(mlc_ctx* c) : ctx(c), buf(&c->ws), cap(&c->ws_bytes) {}
   Workspace(mlc_ctx* c, void** b, size_t* n) : ctx(c), buf(b), cap(n) {}
+  // A buffer of `bytes` at device address `base` (256-byte aligned): commit()
+  // fails with MLC_ERR_ARG when the slices do not fit, and never allocates.
+  Workspace(mlc_ctx* c, void* base, size_t bytes, const char* what)
+      : ctx(c), buf(&fixed_buf), cap(&fixed_cap), fixed(true), fixed_buf(base), fixed_cap(bytes),
+        fixed_what(what) {}
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
   // Declare a slice; returns its index.  Call commit() before get().
   int add(size_t bytes) {
     size_t off = (used + 255) & ~(size_t)255;
@@ -117,6 +129,12 @@ struct Workspace {
     return (int)slices.size() - 1;
   }
   void commit() {
+    if (fixed) {
+      if (used > fixed_cap)
+        fail(MLC_ERR_ARG, std::string(fixed_what) + ": this call needs " + std::to_string(used) +
+                              " bytes of scratch, the static scratch holds " + std::to_string(fixed_cap));
+      return;
+    }
     if (used > *cap) {
       if (*buf) {
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize(workspace)");

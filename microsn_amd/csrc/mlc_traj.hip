@@ -27,6 +27,7 @@
 // mu <= 0 gets chi2 = +inf and a zero gradient.
 #include <cmath>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "mlc_fft.hpp"
@@ -787,7 +788,11 @@ void validate_fused(const Args& a) {
 
 // before_band(p*B + b), when given, runs on the host before the launch of
 // band b of profile p is queued (the caller writes that band's node maps).
-void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* before_band = nullptr) {
+// With `fixed` the call's scratch comes from that buffer instead of the
+// context's workspace (no allocation; too small a buffer is an argument error
+// raised before any launch).
+void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* before_band = nullptr,
+               Workspace* fixed = nullptr) {
   validate_fused(a);
   if (a.ncurves == 0) return;
   const int P = a.P, B = a.B, n = a.ncurves;
@@ -802,7 +807,8 @@ void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* befo
   const int nq = P * pg.C + 1;
   const int64_t m = (int64_t)nq * nblk;
   MLC_REQUIRE(m <= INT32_MAX, "lc_chi2_grad: too many curves x buckets");
-  Workspace ws(ctx);
+  Workspace own(ctx);
+  Workspace& ws = fixed ? *fixed : own;
   const int ibh = ws.add(sizeof(int) * m), ist = ws.add(sizeof(int) * 2 * (P + 1)),
             itot = ws.add(sizeof(int) * 2 * (size_t)nq),
             irec = ws.add(sizeof(CurveRec) * (size_t)n), iacc = ws.add(sizeof(double) * 8 * (size_t)n),
@@ -915,6 +921,33 @@ void run_plain(mlc_ctx* ctx, const Args& a) {
   hip_check(hipFree(samp), "hipFree(samples)");
 }
 
+// ---- allocation-free variant: the scratch of mlc_lc_chi2_grad_noalloc lives
+// in this file-scope device array (one copy per GPU, for the life of the
+// process).  Host code never uses the array's name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress, called
+// once per device with that device current, and is what the kernels get.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // 64 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch (the context's device is current:
+// guarded() made it so).
+void* scratch_ptr(mlc_ctx* ctx) {
+  MLC_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices, "lc_chi2_grad_noalloc: device ordinal too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  void*& p = g_scratch_ptr[ctx->device];
+  if (!p) {
+    void* q = nullptr;
+    hip_check(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    MLC_REQUIRE(q != nullptr, "lc_chi2_grad_noalloc: no device address for the static scratch");
+    p = q;
+  }
+  return p;
+}
+
 }  // namespace
 
 namespace mlc {
@@ -975,6 +1008,21 @@ int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix, v
     };
     if (map->precision == MLC_F64) fft_convolve_nodes<double>(map, Ps, dst.data(), &drive);
     else fft_convolve_nodes<float>(map, Ps, dst.data(), &drive);
+  });
+}
+
+int mlc_lc_chi2_grad_noalloc(mlc_ctx* ctx, int precision, const void* d_nodes, int nnodes, int Nx, int Ny,
+                             int P, int B, const int* node_off, const int* node_cnt,
+                             const double* node_rad, const double* band_scale, int ncurves,
+                             const double* d_params, const int* d_prof, const int* d_obs, int nobs,
+                             const int* d_obs_off, int nepoch, const double* d_t, const double* d_mobs,
+                             const double* d_sig, double* d_chi2, double* d_grad) {
+  return guarded(ctx, [&] {
+    Args a{precision, d_nodes, nnodes, Nx, Ny, P, B, node_off, node_cnt, node_rad, band_scale,
+           ncurves, d_params, d_prof, d_obs, nobs, d_obs_off, nepoch, d_t, d_mobs, d_sig, d_chi2,
+           d_grad};
+    Workspace ws(ctx, scratch_ptr(ctx), kNoAllocScratchBytes, "lc_chi2_grad_noalloc");
+    run_fused(ctx, a, nullptr, &ws);
   });
 }
 
