@@ -315,6 +315,59 @@ int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix,
                               const int* d_obs_off, int nepoch, const double* d_t,
                               const double* d_mobs, const double* d_sig,
                               double* d_chi2, double* d_grad);
+/* Several maps in one call (batched light curves): curve c lies on map
+ * d_map[c] (0 <= d_map[c] < nmaps; another value gives chi2 = +inf), has
+ * profile d_prof[c] and observation set d_obs[c].  Map m is Ny[m] x Nx[m]
+ * (host arrays; multiples of 4).  d_nodes (host array of nnodes device
+ * pointers) gives node map k, tiled like mlc_map_convolve d_node, in
+ * `precision`; the nodes of (map m, profile p, band b) are node_off[j] ..
+ * node_off[j] + node_cnt[j] - 1, j = (m*P + p)*B + b, radii node_rad
+ * ascending (1..16 per job), and a node map serves the jobs of one map
+ * only.  Every other argument means what it means for mlc_lc_chi2_grad.
+ * All bands of all maps run in one persistent launch that takes groups of
+ * curves from a per-XCD work queue (each group on one (map, profile), its
+ * bands in order), after a counting partition of the curves by (map,
+ * profile, map cell).  Each curve's chi2 and gradient are those
+ * mlc_lc_chi2_grad gives on its map alone with the same launch shape (bit
+ * for bit), so maps of different sizes and curves with uneven epoch counts
+ * share one launch.  Generalises the per-profile serial loop of
+ * expanding_source (gpu_functions.cu:180-210) to many maps. */
+int mlc_lc_chi2_grad_batch(mlc_ctx* ctx, int precision, int nmaps, const int* Nx,
+                           const int* Ny, int nnodes, void* const* d_nodes,
+                           const double* node_rad, int P, int B, const int* node_off,
+                           const int* node_cnt, const double* band_scale, int ncurves,
+                           const double* d_params, const int* d_map, const int* d_prof,
+                           const int* d_obs, int nobs, const int* d_obs_off, int nepoch,
+                           const double* d_t, const double* d_mobs, const double* d_sig,
+                           double* d_chi2, double* d_grad);
+
+/* Launch-shape tuner of the band kernels (mlc_lc_chi2_grad and its
+ * variants, mlc_lc_chi2_grad_batch).  A shape is the lanes per curve (8, 16
+ * or 32) and the grid as a multiple of the workgroups the CUs hold at once.
+ * MLC_TUNE_AUTO (the default): the first call with a new key (kernel kind,
+ * precision, log2 of the mean epochs per curve and band, log2 of the
+ * largest map's pixels) times the six candidates on its own inputs and keeps
+ * the fastest for the context; a call under graph capture never tunes (it
+ * uses a kept shape, else 16 lanes and one grid fill).  MLC_TUNE_OFF: always
+ * 16 lanes, one grid fill.  MLC_TUNE_FORCE: the shape of mlc_tune_force.
+ * Shapes differ only in the order of the FP64 per-band sums.  No reference
+ * counterpart (its launch geometry is fixed, gpu_functions.cu:88-91). */
+#define MLC_TUNE_OFF 0
+#define MLC_TUNE_AUTO 1
+#define MLC_TUNE_FORCE 2
+#define MLC_TUNE_KIND_BANDS 0 /* per-(profile, band) launches */
+#define MLC_TUNE_KIND_BATCH 1 /* the persistent batched launch */
+/* Force flag: the multi-node band kernel also for single-node bands. */
+#define MLC_TUNE_GENERAL_NODES 1
+int mlc_tune_mode(mlc_ctx* ctx, int mode);
+int mlc_tune_force(mlc_ctx* ctx, int lanes, int grid_multiple, int flags);
+/* The kept shape for a key (lanes = grid_multiple = 0 when none is kept). */
+int mlc_tune_lookup(mlc_ctx* ctx, int kind, int precision, double epochs_per_band,
+                    double pixels, int* lanes, int* grid_multiple);
+/* Kept shapes to / from a text file (loading adds to the kept shapes). */
+int mlc_tune_save(mlc_ctx* ctx, const char* path);
+int mlc_tune_load(mlc_ctx* ctx, const char* path);
+
 /* mlc_lc_chi2_grad without any device allocation: the same kernels and the
  * same results bit for bit, but the call's scratch (curve records, partition
  * counts, per-curve accumulators, epoch records: about 136 bytes per curve

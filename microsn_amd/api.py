@@ -73,6 +73,32 @@ class Context:
         """Grow the workspace to at least nbytes now (before a capture)."""
         self.check(lib().mlc_ctx_reserve(self.h, int(nbytes)))
 
+    # ---- launch-shape tuner of the band kernels (mlc_tune_*)
+    TUNE_OFF, TUNE_AUTO, TUNE_FORCE = 0, 1, 2
+    TUNE_KIND_BANDS, TUNE_KIND_BATCH = 0, 1
+    TUNE_GENERAL_NODES = 1
+
+    def tune_mode(self, mode):
+        self.check(lib().mlc_tune_mode(self.h, int(mode)))
+
+    def tune_force(self, lanes, grid_multiple=1, flags=0):
+        """Use this launch shape for every band launch (sets TUNE_FORCE)."""
+        self.check(lib().mlc_tune_force(self.h, int(lanes), int(grid_multiple), int(flags)))
+        self.tune_mode(self.TUNE_FORCE)
+
+    def tune_lookup(self, kind, precision, epochs_per_band, pixels):
+        """(lanes, grid_multiple) kept for a key, or (0, 0)."""
+        la, gm = C.c_int(), C.c_int()
+        self.check(lib().mlc_tune_lookup(self.h, int(kind), _prec(precision), float(epochs_per_band),
+                                         float(pixels), C.byref(la), C.byref(gm)))
+        return la.value, gm.value
+
+    def tune_save(self, path):
+        self.check(lib().mlc_tune_save(self.h, str(path).encode()))
+
+    def tune_load(self, path):
+        self.check(lib().mlc_tune_load(self.h, str(path).encode()))
+
     def close(self):
         if getattr(self, "h", None):
             for m in list(getattr(self, "_maps", ())):

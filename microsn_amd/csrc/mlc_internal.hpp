@@ -56,6 +56,12 @@ struct mlc_ctx {
   // valid while the generation is still g
   uint64_t generation = 0;
 
+  // launch-shape tuner of the light-curve band kernels (mlc_tune_*): mode,
+  // the forced shape, and the kept shape (lanes, grid multiple) per key
+  int tune_mode = MLC_TUNE_AUTO;
+  int tune_force_lanes = 16, tune_force_gmul = 1, tune_force_flags = 0;
+  std::map<uint64_t, std::pair<int, int>> tune_cache;
+
   // per-kernel timing
   bool timing = false;
   struct Pending {
@@ -210,7 +216,7 @@ int fill_grid(mlc_ctx* ctx, int per_cu);
 // uploaded when its bytes differ from the previous call's, reused otherwise,
 // so that a repeated call issues no host-to-device copy (and can be captured
 // into a graph once it has run).
-enum ConstSlot { kTableBins = 0, kTableNodeJobs = 1 };
+enum ConstSlot { kTableBins = 0, kTableNodeJobs = 1, kTableBatchJobs = 2, kTableBatchMaps = 3 };
 const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes);
 
 // Workgroups of `threads` lanes of `kernel` resident at once on one CU (the

@@ -145,30 +145,34 @@ constexpr int kPartChunk = 2048;    // curves per partition block
 constexpr int kPartThreads = 1024;  // threads of a partition block (2 curves each)
 constexpr int kMaxBuckets = 8192;   // partition bucket counters live in LDS
 constexpr int kMaxBandNodes = 16;   // radius nodes of one (profile, band), kernel argument
-#ifndef MLC_LC_LANES
-#define MLC_LC_LANES 16
-#endif
-constexpr int kLanes = MLC_LC_LANES;  // lanes per curve in a band launch
-static_assert(kLanes >= 8 && kLanes <= 64 && (kLanes & (kLanes - 1)) == 0, "lanes per curve");
+constexpr int kDefaultLanes = 16;   // lanes per curve when no tuned shape applies
 
 // Buckets of the partition: profile p and a coarse map cell of the curve's
 // position at the middle of its epochs, cells in Morton order, so that
 // curves processed together read nearby parts of the node maps; bucket P*C
 // collects curves with an invalid profile index.
+// With several maps (map != NULL: per-curve map index, msz: per-map Nx, Ny)
+// the bucket's first key is map * P + profile, and the cell grid spans each
+// curve's own map.
 struct PartGeom {
   int P, C, L;       // profiles, cells (4^L), bits per cell axis
-  double sx, sy;     // cells per pixel along x and y
+  double sx, sy;     // cells per pixel along x and y (one map)
   const int* prof;
   const double* params;
   const int* obs;
   int nobs, B, nepoch;
   const int* obs_off;
   const double* tt;
+  int nmaps = 1;               // maps; buckets of (map, profile): NP = nmaps * P
+  const int* map = nullptr;    // per-curve map index, or NULL (one map)
+  const int* msz = nullptr;    // per-map (Nx, Ny)
+  __host__ __device__ int NP() const { return nmaps * P; }
 };
 
 __device__ __forceinline__ int curve_bucket(const PartGeom& g, int c) {
   const int p = g.prof[c];
-  if (!(p >= 0 && p < g.P)) return g.P * g.C;
+  const int mi = g.map ? g.map[c] : 0;
+  if (!(p >= 0 && p < g.P && mi >= 0 && mi < g.nmaps)) return g.NP() * g.C;
   double tm = 0.0;
   const int o = g.obs[c];
   if (o >= 0 && o < g.nobs) {
@@ -178,19 +182,21 @@ __device__ __forceinline__ int curve_bucket(const PartGeom& g, int c) {
   const double* Pc = g.params + (size_t)c * 7;
   double sn, cs;
   sincos(Pc[2], &sn, &cs);
-  const double x = (Pc[0] + Pc[3] * cs * tm) * g.sx, y = (Pc[1] + Pc[3] * sn * tm) * g.sy;
   const int side = 1 << g.L;
+  const double sx = g.map ? (double)side / g.msz[2 * mi] : g.sx;
+  const double sy = g.map ? (double)side / g.msz[2 * mi + 1] : g.sy;
+  const double x = (Pc[0] + Pc[3] * cs * tm) * sx, y = (Pc[1] + Pc[3] * sn * tm) * sy;
   const int cx = (x >= 0.0) ? (int)min(x, (double)(side - 1)) : 0;  // NaN -> 0
   const int cy = (y >= 0.0) ? (int)min(y, (double)(side - 1)) : 0;
   int m = 0;
   for (int bit = 0; bit < g.L; ++bit) m |= (((cx >> bit) & 1) << (2 * bit)) | (((cy >> bit) & 1) << (2 * bit + 1));
-  return p * g.C + m;
+  return (mi * g.P + p) * g.C + m;
 }
 
 // Per-block bucket counts, stored block-major: bh[block * nq + q].
 __global__ __launch_bounds__(kPartThreads) void k_part_count(PartGeom g, int n, int* __restrict__ bh) {
   extern __shared__ int h[];
-  const int nq = g.P * g.C + 1;
+  const int nq = g.NP() * g.C + 1;
   for (int q = threadIdx.x; q < nq; q += kPartThreads) h[q] = 0;
   __syncthreads();
   const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_fill(PartGeom g, int n, c
                                                    CurveRec* __restrict__ rec,
                                                    int* __restrict__ pos_of) {
   extern __shared__ int h[];
-  const int nq = g.P * g.C + 1;
+  const int nq = g.NP() * g.C + 1;
   for (int q = threadIdx.x; q < nq; q += kPartThreads) h[q] = base[q] + bh[(size_t)blockIdx.x * nq + q];
   __syncthreads();
   const int c0 = blockIdx.x * kPartChunk, c1 = min(n, c0 + kPartChunk);
@@ -369,12 +375,204 @@ __device__ __forceinline__ double group_sum8(double (&v)[8], int li) {
   return r;
 }
 
-// One (profile p, band b) launch.  A group of kLanes lanes per curve record
-// (64/kLanes curves per wavefront, grid-stride over the records of bucket
-// p), the band-b epochs strided across the group.  A curve is checked once
-// against the pixel-centre hull at the earliest and latest epoch of the band:
-// its position is monotone in t, so every epoch in between lies inside as
-// well, and a curve outside is flagged and skips the band.  Per epoch:
+// Radius nodes of one (profile, band) as a sampler reads them: nn >= 1
+// ascending radii rad[], 1/(rad[k+1] - rad[k]) in ih[] (FP32 maps), node map
+// k at node(k).
+template <class NodeAt>
+struct NodeView {
+  int nn;
+  double rlo, rhi;
+  const double* rad;
+  const double* ih;
+  NodeAt node;
+};
+
+// One epoch (time te, observed magnitude em, 1/sigma eis) of curve R on the
+// node maps of one (profile, band): adds its eight FP64 terms to s and
+// raises flag when mu is not positive and finite.  The caller has checked
+// that the position lies in the pixel-centre hull (0 <= x <= Nx - 1,
+// 0 <= y <= Ny - 1).  ONE: a single radius node, first-order Taylor step in
+// r, taps through 32-bit offsets from node(0) (the map is below 4 GiB).
+// rowstep = 4 Nx - 12, the pair-offset step from pixel row 4k+3 to 4k+4.
+template <class T, bool ONE, class NodeAt>
+__device__ __forceinline__ void lc_sample(const NodeView<NodeAt>& nv, int Nx, int Ny, int rowstep, double bsc,
+                                          const CurveRec& R, double te, double em, double eis,
+                                          const LnEntry* lnt, double (&s)[8], bool& flag) {
+  constexpr bool kF32 = sizeof(T) == 4;
+  constexpr double kMag = -1.08573620475812959810;  // -2.5 / ln 10
+  constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
+  const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
+  const double r = bsc * fma(R.w, te, R.r0);
+  const int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
+  const T fx = (T)(x - ix), fy = (T)(y - iy), gx = (T)1 - fx, gy = (T)1 - fy;
+  const int o00 = tile32(ix, iy, Nx);
+  const int sx = (ix & 3) == 3 ? 13 : 1, sy = (iy & 3) == 3 ? rowstep : 4;
+  const int o01 = o00 + sx, o10 = o00 + sy, o11 = o10 + sx;
+  // corner values u (mu) and v (dmu/dr) from one or two radius nodes
+  T u[4], v[4];
+  if constexpr (ONE) {  // first-order Taylor step about the only node
+    const T dr = (T)(r - nv.rlo);
+    const char* bb = reinterpret_cast<const char*>(nv.node(0));
+    auto at = [&](int o) -> const cpx<T>* {
+      return reinterpret_cast<const cpx<T>*>(bb + (uint32_t)o * (uint32_t)sizeof(cpx<T>));
+    };
+    cpx<T> c0, c1, c2, c3;
+    if constexpr (kF32) {
+      // the left and right taps of a row are neighbours in the tile unless
+      // ix % 4 == 3: one 16-byte load per row, and a second tap load only on
+      // lanes whose cell crosses a tile column (the 16-byte load stays
+      // inside the map: ix + 1 <= Nx - 1)
+      typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+      const f4a8 top = *reinterpret_cast<const f4a8*>(at(o00));
+      const f4a8 bot = *reinterpret_cast<const f4a8*>(at(o10));
+      c0 = cpx<T>{top.x, top.y};
+      c2 = cpx<T>{bot.x, bot.y};
+      if (sx == 1) {
+        c1 = cpx<T>{top.z, top.w};
+        c3 = cpx<T>{bot.z, bot.w};
+      } else {
+        c1 = *at(o01);
+        c3 = *at(o11);
+      }
+    } else {
+      c0 = *at(o00); c1 = *at(o01); c2 = *at(o10); c3 = *at(o11);
+    }
+    u[0] = c0.x + dr * c0.y; v[0] = c0.y;
+    u[1] = c1.x + dr * c1.y; v[1] = c1.y;
+    u[2] = c2.x + dr * c2.y; v[2] = c2.y;
+    u[3] = c3.x + dr * c3.y; v[3] = c3.y;
+  } else {
+    // node interval k, k+1; Taylor about the end node outside [rlo, rhi].
+    // A NaN radius counts as below the first node (its Taylor step gives a
+    // NaN mu, which flags the curve), and a Taylor step reads node ka twice,
+    // so kb never passes the last node, also when nn == 1.
+    const int nn = nv.nn;
+    int k = 0;
+    while (k < nn - 2 && nv.rad[k + 1] <= r) ++k;
+    const bool lo = !(r > nv.rlo), hi = !lo && r >= nv.rhi;
+    const int ka = hi ? nn - 1 : k, kb = (lo || hi) ? ka : k + 1;
+    const double ra = nv.rad[ka], h = nv.rad[kb] - ra;
+    T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
+    if (ka == kb) {
+      const T dr = (T)(r - ra);
+      wa = 1; wda = dr; wb = 0; wdb = 0;
+      za = 0; zda = 1; zb = 0; zdb = 0;
+    } else {
+      // FP32 maps: 1/h from the table; FP64 maps keep the division
+      const double ih = kF32 ? nv.ih[ka] : 1.0 / h;
+      const T S = (T)(kF32 ? (r - ra) * ih : (r - ra) / h), S2 = S * S, S3 = S2 * S;
+      const T H = (T)h, iH = (T)ih;
+      wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
+      wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
+      za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
+      zb = (-6 * S2 + 6 * S) * iH; zdb = 3 * S2 - 2 * S;
+    }
+    const cpx<T>* ma = nv.node(ka);
+    const cpx<T>* mb = nv.node(kb);
+    const int off[4] = {o00, o01, o10, o11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const cpx<T> A = ma[off[q]], Bq = mb[off[q]];
+      u[q] = wa * A.x + wda * A.y + wb * Bq.x + wdb * Bq.y;
+      v[q] = za * A.x + zda * A.y + zb * Bq.x + zdb * Bq.y;
+    }
+  }
+  const T mu_t = gy * (gx * u[0] + fx * u[1]) + fy * (gx * u[2] + fx * u[3]);
+  const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
+  const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
+  const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
+  flag |= !(mu_t > (T)0 && mu_t < (T)INFINITY);
+  double lnmu, rmu;
+  if constexpr (kF32) {
+    lnmu = ln_f32(mu_t, lnt);
+    rmu = (double)__builtin_amdgcn_rcpf(mu_t);
+  } else {
+    lnmu = ln_pos(mu_t);
+    rmu = 1.0 / mu_t;
+  }
+  const double rho = (em - fma(kMag, lnmu, R.dm)) * eis;
+  const double q = rho * eis;
+  const double dc = kDc * q * rmu, tdc = dc * te;
+  const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
+  s[0] = fma(rho, rho, s[0]);
+  s[1] = fma(dc, mx, s[1]);
+  s[2] = fma(dc, my, s[2]);
+  s[3] = fma(tdc, mx, s[3]);
+  s[4] = fma(tdc, my, s[4]);
+  s[5] = fma(dc, mr, s[5]);
+  s[6] = fma(tdc, mr, s[6]);
+  s[7] += q;
+}
+
+// The band-b epochs [e0, e1) of one curve group, strided over its LANES
+// lanes: from the wavefront's LDS copy `wsep` of one (observation set,
+// band) segment's records when every curve of the wavefront uses the staged
+// segment st = o * B + b (the copy is refreshed first when the wavefront's
+// groups agree on a set whose band-b segment has at most kStageEpochs
+// epochs), otherwise from global memory one record ahead.  st is uniform
+// per wavefront.
+template <class T, bool ONE, int LANES, class NodeAt>
+__device__ __forceinline__ void lc_band_epochs(const NodeView<NodeAt>& nv, int Nx, int Ny, double bsc,
+                                               const CurveRec& R, bool act, bool ok, int e0, int e1, int b,
+                                               int B, int nobs, const int* __restrict__ obs_off, int nepoch,
+                                               const Epoch* __restrict__ ep, Epoch* wsep, int& st_o,
+                                               const LnEntry* lnt, double (&s)[8], bool& flag) {
+  const int lane = threadIdx.x & 63, li = lane % LANES;
+  const int rowstep = 4 * Nx - 12;
+  const int cand = __builtin_amdgcn_readfirstlane(act ? R.o : -1);
+  if (cand >= 0 && cand < nobs && cand * B + b != st_o && __all(!act || R.o == cand)) {
+    const int c0 = obs_off[(size_t)cand * B + b], c1 = obs_off[(size_t)cand * B + b + 1];
+    if (c0 >= 0 && c1 <= nepoch && c0 <= c1 && c1 - c0 <= kStageEpochs) {
+      for (int q = lane; q < c1 - c0; q += 64) wsep[q] = ep[c0 + q];
+      st_o = cand * B + b;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int jend = ok ? e1 : e0;
+  int j = e0 + li;
+  if (__all(!ok || R.o * B + b == st_o)) {
+    for (; j < jend; j += LANES) {
+      const Epoch E = wsep[j - e0];
+      lc_sample<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is, lnt, s, flag);
+    }
+  } else {
+    Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
+    for (; j < jend; j += LANES) {
+      const Epoch E = en;
+      if (j + LANES < jend) en = ep[j + LANES];
+      lc_sample<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is, lnt, s, flag);
+    }
+  }
+}
+
+// Band-b epoch range of curve R and whether it lies in the pixel-centre hull
+// of an Nx x Ny map at the band's earliest and latest epoch (its position
+// is monotone in t, so every epoch in between lies inside as well).
+__device__ __forceinline__ bool lc_band_range(const CurveRec& R, int b, int B, int nobs,
+                                              const int* __restrict__ obs_off, int nepoch,
+                                              const double2* __restrict__ rng, int Nx, int Ny, int& e0,
+                                              int& e1) {
+  const int o = R.o;
+  const bool okobs = o >= 0 && o < nobs;
+  e0 = okobs ? obs_off[(size_t)o * B + b] : 0;
+  e1 = okobs ? obs_off[(size_t)o * B + b + 1] : 0;
+  bool ok = okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1;
+  if (ok && e0 < e1) {
+    const double xmax = (double)(Nx - 1), ymax = (double)(Ny - 1);
+    const double2 tr = rng[(size_t)o * B + b];
+    const double xa = fma(R.vct, tr.x, R.x0), xb = fma(R.vct, tr.y, R.x0);
+    const double ya = fma(R.vst, tr.x, R.y0), yb = fma(R.vst, tr.y, R.y0);
+    ok = xa >= 0.0 && xa <= xmax && xb >= 0.0 && xb <= xmax && ya >= 0.0 && ya <= ymax && yb >= 0.0 &&
+         yb <= ymax;
+  }
+  return ok;
+}
+
+// One (profile p, band b) launch.  A group of LANES lanes per curve record
+// (64/LANES curves per wavefront, grid-stride over the records of bucket
+// p), the band-b epochs strided across the group.  A curve outside the
+// pixel-centre hull at the band's earliest or latest epoch is flagged and
+// skips the band.  Per epoch (lc_sample):
 //   mu, dmu/dx, dmu/dy, dmu/dr  from the bilinear taps of the nodes (T)
 //   rho = (mobs - dm + 2.5 log10 mu) / sig,  q = rho / sig        (FP64)
 //   dc = d chi2 / d mu = 5 q / (ln 10 mu)
@@ -383,19 +581,24 @@ __device__ __forceinline__ double group_sum8(double (&v)[8], int li) {
 //   [5] bsc dc mr  [6] bsc dc t mr  [7] -2 q
 // go to the record's accumulator (band 0 writes, later bands add).  ONE: the
 // (profile, band) has a single radius node (first-order Taylor step in r).
-template <class T, bool FIRST, bool ONE>
+template <class T>
+struct ConsecutiveNodes {
+  const cpx<T>* base;
+  size_t mapsz;
+  __device__ __forceinline__ const cpx<T>* operator()(int k) const { return base + mapsz * k; }
+};
+
+template <class T, bool FIRST, bool ONE, int LANES>
 __global__ __launch_bounds__(256) void k_lc_band(
     const cpx<T>* __restrict__ base, int Nx, int Ny, BandNodes bn, int p, int b, int B,
     const CurveRec* __restrict__ rec, const int* __restrict__ start, const int* __restrict__ cnt,
     int nobs, const int* __restrict__ obs_off, int nepoch, const Epoch* __restrict__ ep,
     const double2* __restrict__ rng, double* __restrict__ acc, int* __restrict__ bad) {
-  constexpr int GPW = 64 / kLanes;
+  constexpr int GPW = 64 / LANES;
   constexpr bool kF32 = sizeof(T) == 4;
-  constexpr double kMag = -1.08573620475812959810;  // -2.5 / ln 10
-  constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
   __shared__ double srad[ONE ? 1 : kMaxBandNodes], sih[ONE ? 1 : kMaxBandNodes];
   __shared__ LnEntry lnt[kF32 ? kLnTab : 1];
-  // per wavefront: the band's epoch records of one observation set (st_o)
+  // per wavefront: the band's epoch records of one observation set
   __shared__ Epoch sep[4][kStageEpochs];
   if constexpr (!ONE) {
     if (threadIdx.x == 0) {
@@ -410,14 +613,12 @@ __global__ __launch_bounds__(256) void k_lc_band(
     for (int q = threadIdx.x; q < kLnTab; q += 256) lnt[q] = ln_entry(q);
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, gi = lane / kLanes, li = lane % kLanes;
+  const int lane = threadIdx.x & 63, gi = lane / LANES, li = lane % LANES;
   const int pos0 = start[p], ncur = cnt[p];
-  const size_t mapsz = (size_t)Nx * Ny;
-  const double bsc = bn.bsc, xmax = (double)(Nx - 1), ymax = (double)(Ny - 1);
-  const int nn = bn.cnt;
-  const double rlo = bn.rad[0], rhi = ONE ? rlo : srad[nn - 1];
-  // one row of 4 x 4 tiles: the pair-offset step from pixel row 4k+3 to 4k+4
-  const int rowstep = 4 * Nx - 12;
+  const double bsc = bn.bsc;
+  const NodeView<ConsecutiveNodes<T>> nv{bn.cnt, bn.rad[0], ONE ? bn.rad[0] : srad[bn.cnt - 1],
+                                         ONE ? nullptr : srad, ONE ? nullptr : sih,
+                                         ConsecutiveNodes<T>{base, (size_t)Nx * Ny}};
   // Workgroups are dispatched round-robin over the 8 XCDs, so blockIdx & 7
   // names the XCD.  Each XCD owns one contiguous eighth of the (map-ordered)
   // record groups and its wavefronts stride through that range together:
@@ -439,163 +640,17 @@ __global__ __launch_bounds__(256) void k_lc_band(
     bool ok = false;
     if (act) {
       R = rec[pos0 + i];
-      const int o = R.o;
-      const bool okobs = o >= 0 && o < nobs;
-      e0 = okobs ? obs_off[(size_t)o * B + b] : 0;
-      e1 = okobs ? obs_off[(size_t)o * B + b + 1] : 0;
-      ok = okobs && e0 >= 0 && e1 <= nepoch && e0 <= e1;
-      if (ok && e0 < e1) {
-        const double2 tr = rng[(size_t)o * B + b];
-        const double xa = fma(R.vct, tr.x, R.x0), xb = fma(R.vct, tr.y, R.x0);
-        const double ya = fma(R.vst, tr.x, R.y0), yb = fma(R.vst, tr.y, R.y0);
-        ok = xa >= 0.0 && xa <= xmax && xb >= 0.0 && xb <= xmax && ya >= 0.0 && ya <= ymax &&
-             yb >= 0.0 && yb <= ymax;
-      }
+      ok = lc_band_range(R, b, B, nobs, obs_off, nepoch, rng, Nx, Ny, e0, e1);
       flag = !ok;
     }
-    // one epoch of this lane's curve: accumulates into s, raises flag
-    auto sample = [&](const double te, const double em, const double eis) {
-      // inside the hull (checked above): 0 <= x <= Nx - 1, 0 <= y <= Ny - 1
-      const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
-      const double r = bsc * fma(R.w, te, R.r0);
-      const int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
-      const T fx = (T)(x - ix), fy = (T)(y - iy), gx = (T)1 - fx, gy = (T)1 - fy;
-      const int o00 = tile32(ix, iy, Nx);
-      const int sx = (ix & 3) == 3 ? 13 : 1, sy = (iy & 3) == 3 ? rowstep : 4;
-      const int o01 = o00 + sx, o10 = o00 + sy, o11 = o10 + sx;
-      // corner values u (mu) and v (dmu/dr) from one or two radius nodes
-      T u[4], v[4];
-      if constexpr (ONE) {  // first-order Taylor step about the only node
-        const T dr = (T)(r - rlo);
-        // 32-bit byte offsets from the uniform map base (the launcher
-        // checks that the map is below 4 GiB)
-        const char* bb = reinterpret_cast<const char*>(base);
-        auto at = [&](int o) -> const cpx<T>* {
-          return reinterpret_cast<const cpx<T>*>(bb + (uint32_t)o * (uint32_t)sizeof(cpx<T>));
-        };
-        cpx<T> c0, c1, c2, c3;
-        if constexpr (kF32) {
-          // the left and right taps of a row are neighbours in the tile
-          // unless ix % 4 == 3: one 16-byte load per row, and a second
-          // tap load only on lanes whose cell crosses a tile column (the
-          // 16-byte load stays inside the map: ix + 1 <= Nx - 1)
-          typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
-          const f4a8 top = *reinterpret_cast<const f4a8*>(at(o00));
-          const f4a8 bot = *reinterpret_cast<const f4a8*>(at(o10));
-          c0 = cpx<T>{top.x, top.y};
-          c2 = cpx<T>{bot.x, bot.y};
-          if (sx == 1) {
-            c1 = cpx<T>{top.z, top.w};
-            c3 = cpx<T>{bot.z, bot.w};
-          } else {
-            c1 = *at(o01);
-            c3 = *at(o11);
-          }
-        } else {
-          c0 = *at(o00); c1 = *at(o01); c2 = *at(o10); c3 = *at(o11);
-        }
-        u[0] = c0.x + dr * c0.y; v[0] = c0.y;
-        u[1] = c1.x + dr * c1.y; v[1] = c1.y;
-        u[2] = c2.x + dr * c2.y; v[2] = c2.y;
-        u[3] = c3.x + dr * c3.y; v[3] = c3.y;
-      } else {
-        // node interval k, k+1; Taylor about the end node outside [rlo, rhi].
-        // A NaN radius counts as below the first node (its Taylor step gives
-        // a NaN mu, which flags the curve), and a Taylor step reads node ka
-        // twice, so kb never passes the last node, also when nn == 1.
-        int k = 0;
-        while (k < nn - 2 && srad[k + 1] <= r) ++k;
-        const bool lo = !(r > rlo), hi = !lo && r >= rhi;
-        const int ka = hi ? nn - 1 : k, kb = (lo || hi) ? ka : k + 1;
-        const double ra = srad[ka], h = srad[kb] - ra;
-        T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
-        if (ka == kb) {
-          const T dr = (T)(r - ra);
-          wa = 1; wda = dr; wb = 0; wdb = 0;
-          za = 0; zda = 1; zb = 0; zdb = 0;
-        } else {
-          // FP32 maps: 1/h from the table; FP64 maps keep the division
-          const double ih = kF32 ? sih[ka] : 1.0 / h;
-          const T S = (T)(kF32 ? (r - ra) * ih : (r - ra) / h), S2 = S * S, S3 = S2 * S;
-          const T H = (T)h, iH = (T)ih;
-          wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
-          wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
-          za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
-          zb = (-6 * S2 + 6 * S) * iH; zdb = 3 * S2 - 2 * S;
-        }
-        const cpx<T>* ma = base + mapsz * ka;
-        const cpx<T>* mb = base + mapsz * kb;
-        const int off[4] = {o00, o01, o10, o11};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const cpx<T> A = ma[off[q]], Bq = mb[off[q]];
-          u[q] = wa * A.x + wda * A.y + wb * Bq.x + wdb * Bq.y;
-          v[q] = za * A.x + zda * A.y + zb * Bq.x + zdb * Bq.y;
-        }
-      }
-      const T mu_t = gy * (gx * u[0] + fx * u[1]) + fy * (gx * u[2] + fx * u[3]);
-      const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
-      const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
-      const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
-      flag |= !(mu_t > (T)0 && mu_t < (T)INFINITY);
-      double lnmu, rmu;
-      if constexpr (kF32) {
-        lnmu = ln_f32(mu_t, lnt);
-        rmu = (double)__builtin_amdgcn_rcpf(mu_t);
-      } else {
-        lnmu = ln_pos(mu_t);
-        rmu = 1.0 / mu_t;
-      }
-      const double rho = (em - fma(kMag, lnmu, R.dm)) * eis;
-      const double q = rho * eis;
-      const double dc = kDc * q * rmu, tdc = dc * te;
-      const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
-      s[0] = fma(rho, rho, s[0]);
-      s[1] = fma(dc, mx, s[1]);
-      s[2] = fma(dc, my, s[2]);
-      s[3] = fma(tdc, mx, s[3]);
-      s[4] = fma(tdc, my, s[4]);
-      s[5] = fma(dc, mr, s[5]);
-      s[6] = fma(tdc, mr, s[6]);
-      s[7] += q;
-    };
-    // the first group's observation set, staged in LDS when it changes,
-    // every active group of the wavefront uses it (otherwise the copy would
-    // not save the groups' own loads) and its band has at most
-    // kStageEpochs epochs: the epoch records then come from LDS instead of
-    // vector-memory loads
-    const int cand = __builtin_amdgcn_readfirstlane(act ? R.o : -1);
-    if (cand >= 0 && cand < nobs && cand != st_o && __all(!act || R.o == cand)) {
-      const int c0 = obs_off[(size_t)cand * B + b], c1 = obs_off[(size_t)cand * B + b + 1];
-      if (c0 >= 0 && c1 <= nepoch && c0 <= c1 && c1 - c0 <= kStageEpochs) {
-        for (int q = lane; q < c1 - c0; q += 64) wsep[q] = ep[c0 + q];
-        st_o = cand;
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // epochs of this lane: j = e0 + li + kLanes k
-    const int jend = ok ? e1 : e0;
-    int j = e0 + li;
-    if (__all(!ok || R.o == st_o)) {
-      for (; j < jend; j += kLanes) {
-        const Epoch E = wsep[j - e0];
-        sample(E.t, E.m, E.is);
-      }
-    } else {
-      // each record read one iteration ahead
-      Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
-      for (; j < jend; j += kLanes) {
-        const Epoch E = en;
-        if (j + kLanes < jend) en = ep[j + kLanes];
-        sample(E.t, E.m, E.is);
-      }
-    }
-    const double val = group_sum8<kLanes>(s, li);
+    lc_band_epochs<T, ONE, LANES>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off, nepoch, ep, wsep,
+                                  st_o, lnt, s, flag);
+    const double val = group_sum8<LANES>(s, li);
     const unsigned long long bm = __ballot(flag);
-    constexpr unsigned long long gmask = kLanes == 64 ? ~0ull : ((1ull << (kLanes & 63)) - 1);
-    const bool gbad = ((bm >> (gi * kLanes)) & gmask) != 0;
-    if (act && li % (kLanes / 8) == 0) {
-      const int vi = li / (kLanes / 8);
+    constexpr unsigned long long gmask = LANES == 64 ? ~0ull : ((1ull << (LANES & 63)) - 1);
+    const bool gbad = ((bm >> (gi * LANES)) & gmask) != 0;
+    if (act && li % (LANES / 8) == 0) {
+      const int vi = li / (LANES / 8);
       const double sc = vi == 7 ? -2.0 : (vi == 5 || vi == 6) ? bsc : 1.0;
       double* a = acc + (size_t)(pos0 + i) * 8 + vi;
       if (FIRST) *a = sc * val;
@@ -603,6 +658,154 @@ __global__ __launch_bounds__(256) void k_lc_band(
       if (vi == 0) {
         if (FIRST) bad[pos0 + i] = gbad;
         else bad[pos0 + i] |= gbad;
+      }
+    }
+  }
+}
+
+// ---- batched path: curves on several maps, all bands in one persistent launch
+// One (map, profile, band) of a batched call: its radius nodes (device
+// addresses of the node maps, radii, reciprocal spacings), the band scale
+// and the map size.  one: a single node on a map below 4 GiB.
+struct alignas(16) BatchJob {
+  const void* node[kMaxBandNodes];
+  double rad[kMaxBandNodes];
+  double ih[kMaxBandNodes];
+  double bsc;
+  int cnt, one, Nx, Ny;
+};
+
+template <class T>
+struct JobNodes {
+  const BatchJob* J;
+  __device__ __forceinline__ const cpx<T>* operator()(int k) const {
+    return reinterpret_cast<const cpx<T>*>(J->node[k]);
+  }
+};
+
+// Work queue of the persistent band kernel: per XCD a counter over its
+// eighth of the record groups, handed out kBatchChunk groups at a time; a
+// wavefront whose XCD's range is used up takes chunks from the other XCDs'
+// counters in turn.  Counters only grow, so every wavefront stops after at
+// most one failed take per counter.
+constexpr int kBatchChunk = 4;
+
+// Record groups of the batched kernel: the records of (map, profile) mp,
+// [start[mp], start[mp] + cnt[mp]), form ceil(cnt[mp] / gpw) groups of gpw
+// records that never straddle two (map, profile) ranges; gstart[mp] is the
+// first group of mp and gstart[nmp] the number of groups (one workgroup).
+// It also zeroes the 8 work-queue counters of the batched kernel.
+__global__ __launch_bounds__(1024) void k_batch_groups(const int* __restrict__ cnt, int nmp, int gpw,
+                                                      int* __restrict__ gstart, int* __restrict__ qctr) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x, seg = (nmp + 1023) / 1024;
+  if (t < 8) qctr[t] = 0;
+  const int a = min(nmp, t * seg), e = min(nmp, a + seg);
+  int sum = 0;
+  for (int i = a; i < e; ++i) sum += (cnt[i] + gpw - 1) / gpw;
+  s[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - sum;
+  for (int i = a; i < e; ++i) {
+    gstart[i] = run;
+    run += (cnt[i] + gpw - 1) / gpw;
+  }
+  if (t == 1023) gstart[nmp] = s[1023];
+}
+
+// All bands of every (map, profile) in one persistent launch.  Records are
+// in bucket order of (map, profile, map cell); a wavefront takes chunks of
+// kBatchChunk record groups from the work queue, and for each group (all of
+// one (map, profile) mp, so the job tables are uniform per wavefront) runs
+// the bands in order, each exactly as k_lc_band does it (the same lanes per
+// curve give the same per-band group sums; band 0 sets the FP64
+// accumulator, later bands add), keeping the accumulator in a register and
+// writing it once.  qctr: 8 counters, zero at launch.
+template <class T, int LANES>
+__global__ __launch_bounds__(256) void k_lc_batch(
+    const BatchJob* __restrict__ jobs, int B, const CurveRec* __restrict__ rec, const int* __restrict__ start,
+    const int* __restrict__ cnt, const int* __restrict__ gstart, int nmp, int nobs,
+    const int* __restrict__ obs_off, int nepoch, const Epoch* __restrict__ ep, const double2* __restrict__ rng,
+    int* __restrict__ qctr, double* __restrict__ acc, int* __restrict__ bad) {
+  constexpr int GPW = 64 / LANES;
+  constexpr bool kF32 = sizeof(T) == 4;
+  __shared__ LnEntry lnt[kF32 ? kLnTab : 1];
+  __shared__ Epoch sep[4][kStageEpochs];
+  if constexpr (kF32) {
+    for (int q = threadIdx.x; q < kLnTab; q += 256) lnt[q] = ln_entry(q);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, gi = lane / LANES, li = lane % LANES;
+  const int ngroups = gstart[nmp];
+  const int nchunks = (ngroups + kBatchChunk - 1) / kBatchChunk;
+  const int xcd = blockIdx.x & 7;
+  Epoch* const wsep = sep[threadIdx.x >> 6];
+  int st_o = -1;  // uniform per wave
+  int mp = 0;     // uniform per wave; groups are taken in increasing order within a chunk
+  for (int hop = 0; hop < 8; ++hop) {  // own XCD's range first, then the others'
+    const int x = (xcd + hop) & 7;
+    const int c0 = (int)((int64_t)nchunks * x / 8), c1 = (int)((int64_t)nchunks * (x + 1) / 8);
+    while (true) {
+      int ch = 0;
+      if (lane == 0) ch = atomicAdd(&qctr[x], 1);
+      ch = __builtin_amdgcn_readfirstlane(__shfl(ch, 0, 64)) + c0;
+      if (ch >= c1) break;  // uniform per wave
+      const int gend = min(ngroups, (ch + 1) * kBatchChunk);
+      for (int w = ch * kBatchChunk; w < gend; ++w) {
+        // the (map, profile) of group w: the last mp with gstart[mp] <= w
+        int lo = 0, hi = nmp - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (gstart[mid] <= w) lo = mid;
+          else hi = mid - 1;
+        }
+        mp = __builtin_amdgcn_readfirstlane(lo);
+        const int k = (w - gstart[mp]) * GPW + gi;
+        const bool act = k < cnt[mp];
+        const int i = start[mp] + k;
+        CurveRec R{};
+        if (act) R = rec[i];
+        double a8 = 0.0;  // accumulator entry vi of this lane's curve
+        bool gbad = false;
+        for (int b = 0; b < B; ++b) {
+          const BatchJob* J = jobs + (size_t)mp * B + b;
+          const int Nx = J->Nx, Ny = J->Ny, cntn = J->cnt;
+          const double bsc = J->bsc;
+          double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          bool flag = false;
+          int e0 = 0, e1 = 0;
+          bool ok = false;
+          if (act) {
+            ok = lc_band_range(R, b, B, nobs, obs_off, nepoch, rng, Nx, Ny, e0, e1);
+            flag = !ok;
+          }
+          const NodeView<JobNodes<T>> nv{cntn, J->rad[0], J->rad[cntn - 1], J->rad, J->ih, JobNodes<T>{J}};
+          if (J->one)
+            lc_band_epochs<T, true, LANES>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off, nepoch,
+                                           ep, wsep, st_o, lnt, s, flag);
+          else
+            lc_band_epochs<T, false, LANES>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off,
+                                            nepoch, ep, wsep, st_o, lnt, s, flag);
+          const double val = group_sum8<LANES>(s, li);
+          const unsigned long long bm = __ballot(flag);
+          constexpr unsigned long long gmask = LANES == 64 ? ~0ull : ((1ull << (LANES & 63)) - 1);
+          gbad |= ((bm >> (gi * LANES)) & gmask) != 0;
+          const int vi = li / (LANES / 8);
+          const double sc = vi == 7 ? -2.0 : (vi == 5 || vi == 6) ? bsc : 1.0;
+          if (b == 0) a8 = sc * val;
+          else a8 += sc * val;
+        }
+        if (act && li % (LANES / 8) == 0) {
+          const int vi = li / (LANES / 8);
+          acc[(size_t)i * 8 + vi] = a8;
+          if (vi == 0) bad[i] = gbad;
+        }
       }
     }
   }
@@ -786,6 +989,108 @@ void validate_fused(const Args& a) {
     MLC_REQUIRE(a.node_cnt[i] <= kMaxBandNodes, "lc_chi2_grad: at most 16 radius nodes per (profile, band)");
 }
 
+// ---- launch-shape tuner of the band kernels
+// A launch shape is the number of lanes per curve (8, 16 or 32: each curve's
+// epochs are strided over that many lanes, 64/lanes curves per wavefront)
+// and the grid in multiples (1 or 2) of the workgroups the CUs hold at once.
+// The best shape depends on epochs per band (short bands waste lanes of wide
+// groups) and map size (cache reuse), so a tuned shape is kept per key
+// (kernel kind, precision, log2 epochs per curve and band, log2 map pixels).
+// In MLC_TUNE_AUTO mode the first call with a new key times every candidate
+// twice on the call's own inputs (HIP events on the context's stream; the
+// second run counts) and keeps the fastest; later calls, and captured calls,
+// reuse it.  A call being captured never tunes: without a kept shape it uses
+// the default.  Every shape gives the same chi2 and gradient up to the
+// order of the FP64 per-band sums (lanes per curve); one shape gives the
+// same bits every time.
+struct LaunchShape {
+  int lanes = kDefaultLanes, gmul = 1;
+};
+enum TuneKind { kTuneBands = MLC_TUNE_KIND_BANDS, kTuneBatch = MLC_TUNE_KIND_BATCH };
+
+template <class F>
+void with_lanes(int lanes, F&& f) {
+  switch (lanes) {
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: fail(MLC_ERR_ARG, "lanes per curve must be 8, 16 or 32");
+  }
+}
+
+int log2_bucket(double v) {
+  if (!(v >= 1.0)) return 0;
+  return std::min(255, (int)std::lround(std::log2(v)));
+}
+
+uint64_t tune_key(int kind, int precision, double epochs_per_band, double pixels) {
+  return ((uint64_t)kind << 24) | ((uint64_t)(precision & 1) << 16) | ((uint64_t)log2_bucket(epochs_per_band) << 8) |
+         (uint64_t)log2_bucket(pixels);
+}
+
+bool capturing(mlc_ctx* ctx) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(ctx->stream, &st), "hipStreamIsCapturing");
+  return st != hipStreamCaptureStatusNone;
+}
+
+bool needs_tuning(mlc_ctx* ctx, uint64_t key) {
+  return ctx->tune_mode == MLC_TUNE_AUTO && !ctx->tune_cache.count(key) && !capturing(ctx);
+}
+
+LaunchShape tuned_shape(mlc_ctx* ctx, uint64_t key) {
+  LaunchShape sh;
+  if (ctx->tune_mode == MLC_TUNE_FORCE) {
+    sh.lanes = ctx->tune_force_lanes;
+    sh.gmul = ctx->tune_force_gmul;
+  } else if (ctx->tune_mode == MLC_TUNE_AUTO) {
+    auto it = ctx->tune_cache.find(key);
+    if (it != ctx->tune_cache.end()) {
+      sh.lanes = it->second.first;
+      sh.gmul = it->second.second;
+    }
+  }
+  return sh;
+}
+
+// Times run(shape) for every candidate and keeps the fastest under `key`.
+LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(LaunchShape)>& run) {
+  struct Restore {  // per-kernel timing is off while the candidates run
+    mlc_ctx* c;
+    bool t;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Restore() {
+      c->timing = t;
+      for (auto x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } rs{ctx, ctx->timing};
+  ctx->timing = false;
+  hip_check(hipEventCreate(&rs.e[0]), "hipEventCreate");
+  hip_check(hipEventCreate(&rs.e[1]), "hipEventCreate");
+  LaunchShape best;
+  float best_ms = INFINITY;
+  for (int lanes : {8, 16, 32})
+    for (int gmul : {1, 2}) {
+      LaunchShape c;
+      c.lanes = lanes;
+      c.gmul = gmul;
+      run(c);  // first run: caches and clocks settle
+      hip_check(hipEventRecord(rs.e[0], ctx->stream), "hipEventRecord");
+      run(c);
+      hip_check(hipEventRecord(rs.e[1], ctx->stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(rs.e[1]), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, rs.e[0], rs.e[1]), "hipEventElapsedTime");
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  ctx->tune_cache[key] = {best.lanes, best.gmul};
+  return best;
+}
+
 // before_band(p*B + b), when given, runs on the host before the launch of
 // band b of profile p is queued (the caller writes that band's node maps).
 // With `fixed` the call's scratch comes from that buffer instead of the
@@ -835,47 +1140,216 @@ void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* befo
   if (a.nepoch > 0)
     MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
                a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
-  constexpr int kCurvesPerBlock = 4 * (64 / kLanes);
-  // as many workgroups as the CUs hold at once (the kernel's occupancy), a
-  // multiple of 8 so that every XCD gets the same number
-  auto grid_for = [&](const void* k) {
-    return (int)(cdiv(std::min<int64_t>(cdiv(n, kCurvesPerBlock), fill_grid(ctx, resident_blocks(k, 256))), 8) *
-                 8);
-  };
   const size_t mapsz = (size_t)a.Nx * a.Ny;
   const size_t esz = a.precision == MLC_F64 ? 2 * sizeof(double) : 2 * sizeof(float);
-  for (int p = 0; p < P; ++p)
-    for (int b = 0; b < B; ++b) {
-      const int pb = p * B + b;
-      BandNodes bn{};
-      bn.cnt = a.node_cnt[pb];
-      bn.bsc = a.band_scale[b];
-      for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
-      const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
-      if (before_band) (*before_band)(pb);
+  const bool force_general = ctx->tune_mode == 2 && (ctx->tune_force_flags & MLC_TUNE_GENERAL_NODES);
+  // every (profile, band) launch with `lanes` lanes per curve and gmul times
+  // as many workgroups as the CUs hold at once (a multiple of 8, so that
+  // every XCD gets the same number)
+  auto launch_bands = [&](int lanes, int gmul, const std::function<void(int)>* before) {
+    with_lanes(lanes, [&](auto lc) {
+      constexpr int L = decltype(lc)::value;
+      auto grid_for = [&](const void* k) {
+        const int64_t want = cdiv(n, 4 * (64 / L));
+        return (int)(cdiv(std::min<int64_t>(want, (int64_t)gmul * fill_grid(ctx, resident_blocks(k, 256))), 8) *
+                     8);
+      };
+      for (int p = 0; p < P; ++p)
+        for (int b = 0; b < B; ++b) {
+          const int pb = p * B + b;
+          BandNodes bn{};
+          bn.cnt = a.node_cnt[pb];
+          bn.bsc = a.band_scale[b];
+          for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
+          const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
+          if (before) (*before)(pb);
 #define MLC_BAND(TT, F, O)                                                                          \
-  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O>),                                            \
-             dim3(grid_for(reinterpret_cast<const void*>(&k_lc_band<TT, F, O>))), dim3(256), 0,      \
+  MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O, L>),                                         \
+             dim3(grid_for(reinterpret_cast<const void*>(&k_lc_band<TT, F, O, L>))), dim3(256), 0,   \
              (const cpx<TT>*)base, a.Nx, a.Ny, bn, p, b, B, rec, start, cnt, a.nobs, a.obs_off,      \
              a.nepoch, ep, rng, acc, bad)
-#define MLC_BAND_T(TT)                          \
-  if (bn.cnt == 1 && mapsz * 2 * sizeof(TT) < ((size_t)1 << 32)) { \
-    if (b == 0) MLC_BAND(TT, true, true);       \
-    else MLC_BAND(TT, false, true);             \
-  } else {                                      \
-    if (b == 0) MLC_BAND(TT, true, false);      \
-    else MLC_BAND(TT, false, false);            \
+#define MLC_BAND_T(TT)                                                                  \
+  if (bn.cnt == 1 && !force_general && mapsz * 2 * sizeof(TT) < ((size_t)1 << 32)) {    \
+    if (b == 0) MLC_BAND(TT, true, true);                                               \
+    else MLC_BAND(TT, false, true);                                                     \
+  } else {                                                                              \
+    if (b == 0) MLC_BAND(TT, true, false);                                              \
+    else MLC_BAND(TT, false, false);                                                    \
   }
-      if (a.precision == MLC_F64) {
-        MLC_BAND_T(double)
-      } else {
-        MLC_BAND_T(float)
-      }
+          if (a.precision == MLC_F64) {
+            MLC_BAND_T(double)
+          } else {
+            MLC_BAND_T(float)
+          }
 #undef MLC_BAND_T
 #undef MLC_BAND
-    }
+        }
+    });
+  };
+  const uint64_t key = tune_key(kTuneBands, a.precision, (double)a.nepoch / ((double)a.nobs * B), mapsz);
+  LaunchShape sh;
+  if (!fixed && needs_tuning(ctx, key)) {
+    // (the allocation-free variant never tunes: it uses a kept shape or the
+    // default)
+    // every node map must exist before the candidates run: the caller's
+    // per-band work (node convolutions) is done first, in band order
+    if (before_band)
+      for (int pb = 0; pb < P * B; ++pb) (*before_band)(pb);
+    sh = tune_pick(ctx, key, [&](LaunchShape c) { launch_bands(c.lanes, c.gmul, nullptr); });
+    before_band = nullptr;
+  } else {
+    sh = tuned_shape(ctx, key);
+  }
+  launch_bands(sh.lanes, sh.gmul, before_band);
   MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, P, start, pos_of,
              a.params, acc, bad, a.chi2, a.grad);
+}
+
+// ---- batched call
+struct BatchArgs {
+  int precision;
+  int nmaps;
+  const int *Nx, *Ny;
+  int nnodes;
+  const void* const* nodes;
+  const double* node_rad;
+  int P, B;
+  const int *node_off, *node_cnt;
+  const double* band_scale;
+  int ncurves;
+  const double* params;
+  const int *map, *prof, *obs;
+  int nobs;
+  const int* obs_off;
+  int nepoch;
+  const double *t, *mobs, *sig;
+  double *chi2, *grad;
+};
+
+void run_batch(mlc_ctx* ctx, const BatchArgs& a) {
+  MLC_REQUIRE(a.precision == MLC_F32 || a.precision == MLC_F64, "lc_chi2_grad_batch: bad precision");
+  MLC_REQUIRE(a.nmaps >= 1 && a.P >= 1 && a.B >= 1 && a.nnodes >= 1 && a.ncurves >= 0 && a.nobs >= 1 &&
+                  a.nepoch >= 0,
+              "lc_chi2_grad_batch: bad sizes");
+  MLC_REQUIRE(a.Nx && a.Ny && a.nodes && a.node_rad && a.node_off && a.node_cnt && a.band_scale,
+              "lc_chi2_grad_batch: null table");
+  MLC_REQUIRE(a.ncurves == 0 || (a.params && a.map && a.prof && a.obs && a.obs_off && a.chi2 && a.grad),
+              "lc_chi2_grad_batch: null curve arrays");
+  MLC_REQUIRE(a.nepoch == 0 || (a.t && a.mobs && a.sig), "lc_chi2_grad_batch: null epoch arrays");
+  const int P = a.P, B = a.B, NP = a.nmaps * P, n = a.ncurves;
+  MLC_REQUIRE((int64_t)NP * B <= 65536, "lc_chi2_grad_batch: at most 65536 (map, profile, band) jobs");
+  size_t max_pix = 0;
+  for (int m = 0; m < a.nmaps; ++m) {
+    MLC_REQUIRE(a.Nx[m] >= 4 && a.Ny[m] >= 4 && a.Nx[m] % 4 == 0 && a.Ny[m] % 4 == 0,
+                "lc_chi2_grad_batch: node maps need Nx, Ny >= 4 and divisible by 4");
+    MLC_REQUIRE((int64_t)a.Nx[m] * a.Ny[m] <= INT32_MAX, "lc_chi2_grad_batch: map too large for 32-bit offsets");
+    max_pix = std::max(max_pix, (size_t)a.Nx[m] * a.Ny[m]);
+  }
+  for (int k = 0; k < a.nnodes; ++k) MLC_REQUIRE(a.nodes[k], "lc_chi2_grad_batch: null node map");
+  const size_t esz = a.precision == MLC_F64 ? 2 * sizeof(double) : 2 * sizeof(float);
+  const bool force_general = ctx->tune_mode == MLC_TUNE_FORCE && (ctx->tune_force_flags & MLC_TUNE_GENERAL_NODES);
+  // the job table; a node map serves the jobs of one map only
+  std::vector<int> owner(a.nnodes, -1);
+  std::vector<BatchJob> jobs((size_t)NP * B);
+  for (int m = 0; m < a.nmaps; ++m)
+    for (int pb = 0; pb < P * B; ++pb) {
+      const int j = m * P * B + pb, b = pb % B;
+      const int off = a.node_off[j], c = a.node_cnt[j];
+      MLC_REQUIRE(c >= 1 && c <= kMaxBandNodes && off >= 0 && off + c <= a.nnodes,
+                  "lc_chi2_grad_batch: node table entry out of range (1..16 nodes per job)");
+      BatchJob J{};
+      J.cnt = c;
+      J.Nx = a.Nx[m];
+      J.Ny = a.Ny[m];
+      J.bsc = a.band_scale[b];
+      J.one = c == 1 && !force_general && (size_t)J.Nx * J.Ny * esz < ((size_t)1 << 32);
+      for (int k = 0; k < c; ++k) {
+        MLC_REQUIRE(owner[off + k] == -1 || owner[off + k] == m,
+                    "lc_chi2_grad_batch: a node map is used by jobs of two maps");
+        owner[off + k] = m;
+        J.node[k] = a.nodes[off + k];
+        J.rad[k] = a.node_rad[off + k];
+        if (k > 0)
+          MLC_REQUIRE(J.rad[k] > J.rad[k - 1], "lc_chi2_grad_batch: node radii must increase");
+      }
+      for (int k = 0; k + 1 < c; ++k) J.ih[k] = 1.0 / (J.rad[k + 1] - J.rad[k]);
+      jobs[j] = J;
+    }
+  std::vector<int> msz(2 * (size_t)a.nmaps);
+  for (int m = 0; m < a.nmaps; ++m) {
+    msz[2 * m] = a.Nx[m];
+    msz[2 * m + 1] = a.Ny[m];
+  }
+  if (n == 0) return;
+  PartGeom pg{P, 1, 0, 0.0, 0.0, a.prof, a.params, a.obs, a.nobs, B, a.nepoch, a.obs_off, a.t};
+  pg.nmaps = a.nmaps;
+  pg.map = a.map;
+  MLC_REQUIRE(NP < kMaxBuckets, "lc_chi2_grad_batch: at most 8191 (map, profile) pairs");
+  while (pg.L < 5 && (int64_t)NP * pg.C * 4 + 1 <= kMaxBuckets) {
+    ++pg.L;
+    pg.C *= 4;
+  }
+  const int nblk = (int)cdiv(n, kPartChunk);
+  const int nq = NP * pg.C + 1;
+  const int64_t mq = (int64_t)nq * nblk;
+  MLC_REQUIRE(mq <= INT32_MAX, "lc_chi2_grad_batch: too many curves x buckets");
+  const BatchJob* d_jobs =
+      (const BatchJob*)const_table(ctx, kTableBatchJobs, jobs.data(), sizeof(BatchJob) * jobs.size());
+  pg.msz = (const int*)const_table(ctx, kTableBatchMaps, msz.data(), sizeof(int) * msz.size());
+  Workspace ws(ctx);
+  const int ibh = ws.add(sizeof(int) * mq), ist = ws.add(sizeof(int) * 2 * (NP + 1)),
+            itot = ws.add(sizeof(int) * 2 * (size_t)nq), irec = ws.add(sizeof(CurveRec) * (size_t)n),
+            iacc = ws.add(sizeof(double) * 8 * (size_t)n), ibad = ws.add(sizeof(int) * (size_t)n),
+            iep = ws.add(sizeof(Epoch) * std::max(a.nepoch, 1)),
+            irng = ws.add(sizeof(double2) * (size_t)a.nobs * B), ipos = ws.add(sizeof(int) * (size_t)n),
+            igs = ws.add(sizeof(int) * (NP + 1)), iq = ws.add(sizeof(int) * 8);
+  ws.commit();
+  int* bh = ws.get<int>(ibh);
+  int *start = ws.get<int>(ist), *cnt = start + (NP + 1);
+  int *tot = ws.get<int>(itot), *qbase = tot + nq;
+  CurveRec* rec = ws.get<CurveRec>(irec);
+  double* acc = ws.get<double>(iacc);
+  int* bad = ws.get<int>(ibad);
+  Epoch* ep = ws.get<Epoch>(iep);
+  double2* rng = ws.get<double2>(irng);
+  int* pos_of = ws.get<int>(ipos);
+  int* gstart = ws.get<int>(igs);
+  int* qctr = ws.get<int>(iq);
+  const size_t hl = sizeof(int) * nq;
+  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, kColScanQ)), dim3(kColScanQ * kColScanSeg),
+             0, bh, nq, nblk, tot);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_base, dim3(1), dim3(1024), 0, tot, nq, NP, pg.C, n, qbase, start,
+             cnt);
+  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh, qbase, rec,
+             pos_of);
+  if (a.nepoch > 0)
+    MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
+               a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
+  auto launch = [&](LaunchShape sh) {
+    with_lanes(sh.lanes, [&](auto lc) {
+      constexpr int L = decltype(lc)::value;
+      MLC_LAUNCH(ctx, "lc_batch_groups", k_batch_groups, dim3(1), dim3(1024), 0, cnt, NP, 64 / L, gstart, qctr);
+      auto go = [&](auto tag) {
+        using TT = decltype(tag);
+        auto k = k_lc_batch<TT, L>;
+        const int64_t want = cdiv(n, 4 * (64 / L));
+        const int grid = (int)(cdiv(std::min<int64_t>(want, (int64_t)sh.gmul *
+                                                                fill_grid(ctx, resident_blocks((const void*)k, 256))),
+                                    8) *
+                               8);
+        MLC_LAUNCH(ctx, "lc_batch", k, dim3(grid), dim3(256), 0, d_jobs, B, rec, start, cnt, gstart, NP, a.nobs,
+                   a.obs_off, a.nepoch, ep, rng, qctr, acc, bad);
+      };
+      if (a.precision == MLC_F64) go(double{});
+      else go(float{});
+    });
+  };
+  const uint64_t key = tune_key(kTuneBatch, a.precision, (double)a.nepoch / ((double)a.nobs * B), (double)max_pix);
+  if (needs_tuning(ctx, key)) tune_pick(ctx, key, launch);
+  launch(tuned_shape(ctx, key));
+  MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, NP, start, pos_of, a.params,
+             acc, bad, a.chi2, a.grad);
 }
 
 void run_plain(mlc_ctx* ctx, const Args& a) {
@@ -1023,6 +1497,86 @@ int mlc_lc_chi2_grad_noalloc(mlc_ctx* ctx, int precision, const void* d_nodes, i
            d_grad};
     Workspace ws(ctx, scratch_ptr(ctx), kNoAllocScratchBytes, "lc_chi2_grad_noalloc");
     run_fused(ctx, a, nullptr, &ws);
+  });
+}
+
+int mlc_lc_chi2_grad_batch(mlc_ctx* ctx, int precision, int nmaps, const int* Nx, const int* Ny, int nnodes,
+                           void* const* d_nodes, const double* node_rad, int P, int B, const int* node_off,
+                           const int* node_cnt, const double* band_scale, int ncurves, const double* d_params,
+                           const int* d_map, const int* d_prof, const int* d_obs, int nobs, const int* d_obs_off,
+                           int nepoch, const double* d_t, const double* d_mobs, const double* d_sig,
+                           double* d_chi2, double* d_grad) {
+  return guarded(ctx, [&] {
+    BatchArgs a{precision, nmaps, Nx, Ny, nnodes, (const void* const*)d_nodes, node_rad, P, B, node_off,
+                node_cnt, band_scale, ncurves, d_params, d_map, d_prof, d_obs, nobs, d_obs_off, nepoch, d_t,
+                d_mobs, d_sig, d_chi2, d_grad};
+    run_batch(ctx, a);
+  });
+}
+
+int mlc_tune_mode(mlc_ctx* ctx, int mode) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(mode == MLC_TUNE_OFF || mode == MLC_TUNE_AUTO || mode == MLC_TUNE_FORCE, "mlc_tune_mode: bad mode");
+    ctx->tune_mode = mode;
+  });
+}
+
+int mlc_tune_force(mlc_ctx* ctx, int lanes, int gmul, int flags) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(lanes == 8 || lanes == 16 || lanes == 32, "mlc_tune_force: lanes must be 8, 16 or 32");
+    MLC_REQUIRE(gmul >= 1 && gmul <= 8, "mlc_tune_force: grid multiple must be 1..8");
+    MLC_REQUIRE((flags & ~MLC_TUNE_GENERAL_NODES) == 0, "mlc_tune_force: unknown flag");
+    ctx->tune_force_lanes = lanes;
+    ctx->tune_force_gmul = gmul;
+    ctx->tune_force_flags = flags;
+  });
+}
+
+int mlc_tune_lookup(mlc_ctx* ctx, int kind, int precision, double epochs_per_band, double pixels, int* lanes,
+                    int* gmul) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(lanes && gmul, "mlc_tune_lookup: null output");
+    MLC_REQUIRE(kind == MLC_TUNE_KIND_BANDS || kind == MLC_TUNE_KIND_BATCH, "mlc_tune_lookup: bad kind");
+    auto it = ctx->tune_cache.find(tune_key(kind, precision, epochs_per_band, pixels));
+    *lanes = it == ctx->tune_cache.end() ? 0 : it->second.first;
+    *gmul = it == ctx->tune_cache.end() ? 0 : it->second.second;
+  });
+}
+
+int mlc_tune_save(mlc_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(path, "mlc_tune_save: null path");
+    FILE* f = std::fopen(path, "w");
+    MLC_REQUIRE(f, std::string("mlc_tune_save: cannot write ") + path);
+    std::fprintf(f, "# mlc launch shapes: key lanes grid_multiple\n");
+    for (auto& kv : ctx->tune_cache)
+      std::fprintf(f, "%llu %d %d\n", (unsigned long long)kv.first, kv.second.first, kv.second.second);
+    const bool bad = std::fclose(f) != 0;
+    MLC_REQUIRE(!bad, std::string("mlc_tune_save: write failed: ") + path);
+  });
+}
+
+int mlc_tune_load(mlc_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] {
+    MLC_REQUIRE(path, "mlc_tune_load: null path");
+    FILE* f = std::fopen(path, "r");
+    MLC_REQUIRE(f, std::string("mlc_tune_load: cannot read ") + path);
+    std::map<uint64_t, std::pair<int, int>> got;
+    char line[256];
+    bool ok = true;
+    while (std::fgets(line, sizeof line, f)) {
+      if (line[0] == '#' || line[0] == '\n') continue;
+      unsigned long long k = 0;
+      int l = 0, g = 0;
+      if (std::sscanf(line, "%llu %d %d", &k, &l, &g) != 3 || !(l == 8 || l == 16 || l == 32) || g < 1 || g > 8) {
+        ok = false;
+        break;
+      }
+      got[(uint64_t)k] = {l, g};
+    }
+    std::fclose(f);
+    MLC_REQUIRE(ok, std::string("mlc_tune_load: malformed line in ") + path);
+    for (auto& kv : got) ctx->tune_cache[kv.first] = kv.second;
   });
 }
 

@@ -143,6 +143,61 @@ def chi2_grad(ctx, nodes: NodeSet, obs: Observations, params, prof, obs_idx, pla
     return chi2, grad
 
 
+class NodeBatch:
+    """The node maps of several maps as one batched call sees them (host
+    tables of mlc_lc_chi2_grad_batch): map m is nodesets[m].  Every NodeSet
+    has the same profile count P, bands B, band scales and precision; the
+    maps may differ in size.  Holds references to the node sets."""
+
+    def __init__(self, nodesets):
+        self.nodesets = list(nodesets)
+        first = self.nodesets[0]
+        self.P, self.B, self.prec = first.P, first.B, first.prec
+        for ns in self.nodesets:
+            if (ns.P, ns.B, ns.prec) != (self.P, self.B, self.prec) or \
+                    not np.array_equal(ns.band_scale, first.band_scale):
+                raise ValueError("NodeBatch: node sets differ in profiles, bands, band scales or precision")
+        self.Nx = np.array([ns.Nx for ns in self.nodesets], np.int32)
+        self.Ny = np.array([ns.Ny for ns in self.nodesets], np.int32)
+        ptrs, rad, off, cnt = [], [], [], []
+        for ns in self.nodesets:
+            base = len(ptrs)
+            ptrs.extend(ns.nodes[k].data_ptr() for k in range(ns.nodes.shape[0]))
+            rad.extend(ns.node_rad.tolist())
+            off.extend((ns.node_off + base).tolist())
+            cnt.extend(ns.node_cnt.tolist())
+        self.nnodes = len(ptrs)
+        self.dptr = (C.c_void_p * len(ptrs))(*ptrs)
+        self.node_rad = np.array(rad, np.float64)
+        self.node_off = np.array(off, np.int32)
+        self.node_cnt = np.array(cnt, np.int32)
+        self.band_scale = np.ascontiguousarray(first.band_scale, np.float64)
+
+    def compute(self):
+        for ns in self.nodesets:
+            ns.compute()
+
+
+def chi2_grad_batch(ctx, batch: NodeBatch, obs: Observations, params, map_idx, prof, obs_idx):
+    """(chi2 [n], grad [n, 7]) for curves on several maps in one call
+    (mlc_lc_chi2_grad_batch): curve c lies on map map_idx[c] of the batch."""
+    _dev(ctx, params, torch.float64, "params")
+    _dev(ctx, map_idx, torch.int32, "map_idx")
+    _dev(ctx, prof, torch.int32, "prof")
+    _dev(ctx, obs_idx, torch.int32, "obs")
+    n = params.shape[0]
+    assert params.shape == (n, NPARAM) and map_idx.shape == prof.shape == obs_idx.shape == (n,)
+    b = batch
+    chi2 = torch.empty(n, dtype=torch.float64, device=ctx.device)
+    grad = torch.empty((n, NPARAM), dtype=torch.float64, device=ctx.device)
+    ctx.check(lib().mlc_lc_chi2_grad_batch(
+        ctx.h, b.prec, len(b.nodesets), b.Nx.ctypes.data, b.Ny.ctypes.data, b.nnodes, C.addressof(b.dptr),
+        b.node_rad.ctypes.data, b.P, b.B, b.node_off.ctypes.data, b.node_cnt.ctypes.data, b.band_scale.ctypes.data,
+        n, ptr(params), ptr(map_idx), ptr(prof), ptr(obs_idx), obs.nobs, ptr(obs.off), obs.nepoch, ptr(obs.t),
+        ptr(obs.mobs), ptr(obs.sig), ptr(chi2), ptr(grad)))
+    return chi2, grad
+
+
 class LightCurveChi2(torch.autograd.Function):
     """chi2 per curve as a differentiable function of the curve parameters.
 
