@@ -204,6 +204,149 @@ def time_graph(ctx, g, gen, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+# ---- the batching (64 maps) and scale (10000^2) workloads: bench extras,
+# also run by the configuration tests
+SIZES3 = (1024, 1280, 1536, 2048, 2560, 3072, 4096)
+
+
+def lognormal_map(ctx, n, seed, mean=5.0, smooth_sigma=3.0):
+    """exp(0.5 g), g ~ N(0, 1) on the device (seeded), smoothed with a
+    Gaussian of smooth_sigma px (FP64 convolution of the library), scaled to
+    the given mean."""
+    import torch
+
+    import microsn_amd.api as api
+
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    g = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen)
+    s = api.MapSpectrum(ctx, torch.exp(0.5 * g), "f64")
+    m = s.convolve("gaussian", smooth_sigma * np.sqrt(2 * np.log(2)))
+    s.close()
+    return (m * (mean / m.mean())).contiguous()
+
+
+def trajectories(rng, n, size, length, tspan, margin):
+    """Straight trajectories of the given length over tspan, origins uniform
+    at least `margin` px inside a size x size map, angles uniform."""
+    x0 = rng.uniform(margin, size - 1 - margin, n)
+    y0 = rng.uniform(margin, size - 1 - margin, n)
+    th = rng.uniform(0, 2 * np.pi, n)
+    return np.stack([x0, y0, th, np.full(n, length / tspan), np.zeros(n), np.zeros(n), rng.normal(0, 0.05, n)], 1)
+
+
+def config3_workload(ctx, prec="f32", ncur=2048, seed=3003):
+    """64 maps (16 systems x 4 images) of 1024^2..4096^2, kappa and gamma
+    uniform in 0.2..0.6 with (1-k)^2 - g^2 > 0.05 (map mean 1/|(1-k)^2-g^2|),
+    flux ratios log-uniform in 0.1..1 as each image's magnitude offset, a
+    Gaussian source of 2 px, ncur trajectories per map, each curve with one
+    of 64 observation sets of 20..400 irregular epochs."""
+    import torch
+
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    rng = np.random.default_rng(seed)
+    nsys, nimg, nsets = 16, 4, 64
+    sets = []
+    for _ in range(nsets):
+        ne = int(rng.integers(20, 401))
+        t = np.sort(rng.uniform(0, 100.0, ne))
+        sets.append([(t, rng.normal(-1.0, 0.3, ne), np.full(ne, 0.05))])
+    obs = lcm.Observations(ctx, 1, sets)
+    maps, specs, nodesets, prm, midx, oidx = [], [], [], [], [], []
+    for s in range(nsys):
+        ratios = np.exp(rng.uniform(np.log(0.1), np.log(1.0), nimg))
+        for i in range(nimg):
+            n = int(rng.choice(SIZES3))
+            while True:
+                k, g = rng.uniform(0.2, 0.6, 2)
+                if (1 - k) ** 2 - g ** 2 > 0.05:
+                    break
+            m = lognormal_map(ctx, n, 3100 + s * nimg + i, mean=1.0 / abs((1 - k) ** 2 - g ** 2))
+            spec = api.MapSpectrum(ctx, m, prec)
+            maps.append(m)
+            specs.append(spec)
+            nodesets.append(lcm.NodeSet(ctx, spec, ["gaussian"], [[[2.0]]], [1.0], compute=False))
+            p = trajectories(rng, ncur, n, 200.0, 100.0, 230.0)
+            p[:, 4] = 2.0
+            p[:, 6] = -2.5 * np.log10(ratios[i])
+            prm.append(p)
+            midx.append(np.full(ncur, len(maps) - 1, np.int32))
+            oidx.append(rng.integers(0, nsets, ncur).astype(np.int32))
+    t = lambda a: torch.tensor(np.concatenate(a), device="cuda")  # noqa: E731
+    return dict(maps=maps, specs=specs, nodesets=nodesets, batch=lcm.NodeBatch(nodesets), obs=obs,
+                params=t(prm), map_idx=t(midx), prof=torch.zeros(len(maps) * ncur, dtype=torch.int32, device="cuda"),
+                obs_idx=t(oidx), bins=api.mpd_bins(100, 1.0, 1e7))
+
+
+def config4_workload(ctx, prec="f32", ntraj=1_000_000, seed=4004):
+    """Four 10000^2 maps (one per lensed image), uniform-disc sources of
+    1, 2.5, 5, 10 and 20 px in five bands of 40 epochs each (200 per curve,
+    0.02 mag noise), ntraj trajectories on every image (4 ntraj curves), a
+    1000-bin table of all chi2."""
+    import torch
+
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    n, nimg, nb, E = 10000, 4, 5, 40
+    radii = [1.0, 2.5, 5.0, 10.0, 20.0]
+    rng = np.random.default_rng(seed)
+    times = [np.sort(rng.uniform(0, 100.0, E)) for _ in range(nb)]
+    obs = lcm.Observations(ctx, nb, [[(tb, rng.normal(-1.7, 0.02, E), np.full(E, 0.02)) for tb in times]])
+    prm = trajectories(rng, ntraj, n, 200.0, 100.0, 230.0)
+    prm[:, 4] = 1.0
+    maps, specs, nodesets = [], [], []
+    for img in range(nimg):
+        m = lognormal_map(ctx, n, 4100 + img)
+        spec = api.MapSpectrum(ctx, m, prec)
+        maps.append(m)
+        specs.append(spec)
+        nodesets.append(lcm.NodeSet(ctx, spec, ["uniform"], [[[r] for r in radii]], radii, compute=False))
+    P = torch.tensor(np.tile(prm, (nimg, 1)), device="cuda")
+    M = torch.tensor(np.repeat(np.arange(nimg, dtype=np.int32), ntraj), device="cuda")
+    z = torch.zeros(nimg * ntraj, dtype=torch.int32, device="cuda")
+    return dict(maps=maps, specs=specs, nodesets=nodesets, batch=lcm.NodeBatch(nodesets), obs=obs, params=P,
+                map_idx=M, prof=z, obs_idx=z, bins=api.mpd_bins(1000, 1.0, 1e7), radii=radii, ntraj=ntraj)
+
+
+def batch_step(ctx, w):
+    """One step of a batched workload: every map's spectrum and node maps,
+    the chi2 + adjoint of all curves in one batched call, the stable sort of
+    the chi2 values and their bin table, with no host synchronisation."""
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    for spec, m, ns in zip(w["specs"], w["maps"], w["nodesets"]):
+        spec.update(m)
+        ns.compute()
+    c, g = lcm.chi2_grad_batch(ctx, w["batch"], w["obs"], w["params"], w["map_idx"], w["prof"], w["obs_idx"])
+    perm, _, sk = api.sort_f64(ctx, c, sorted_keys=True)
+    lo, npb, bc, be = api.bin_table_device(ctx, sk, w["bins"])
+    return c, g, perm, sk, lo, npb, bc, be
+
+
+def capture(ctx, fn):
+    """fn() once on a side stream the library is pointed at (workspace,
+    tables and kept launch shapes settle), then captured into a HIP graph.
+    Returns (graph, the outputs its replays write, context generation)."""
+    import torch
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    ctx.set_stream(s)
+    try:
+        with torch.cuda.stream(s):
+            fn()
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            out = fn()
+    finally:
+        ctx.set_stream(torch.cuda.current_stream())
+    return g, out, ctx.generation()
+
+
 def cpu_model():
     """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
     try:

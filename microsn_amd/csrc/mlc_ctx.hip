@@ -36,7 +36,17 @@ int fill_grid(mlc_ctx* ctx, int per_cu) {
 }
 
 const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes) {
-  auto& t = ctx->tables[slot];
+  return const_table_in(ctx, ctx->tables[slot], host, bytes);
+}
+
+void const_table_free(mlc_ctx::ConstTable& t) {
+  if (t.dev) (void)hipFree(t.dev);
+  t.dev = nullptr;
+  t.cap = 0;
+  t.host.clear();
+}
+
+const void* const_table_in(mlc_ctx* ctx, mlc_ctx::ConstTable& t, const void* host, size_t bytes) {
   if (t.dev && t.host.size() == bytes && std::memcmp(t.host.data(), host, bytes) == 0) return t.dev;
   if (bytes > t.cap) {
     if (t.dev) {  // kernels queued earlier may still read it

@@ -119,6 +119,7 @@ void mlc_map_destroy(mlc_map* m) {
   (void)hipFree(m->g1);
   (void)hipFree(m->ntab);
   (void)hipFree(m->mean);
+  const_table_free(m->jobs);
   delete m;
 }
 

@@ -24,6 +24,7 @@ struct mlc_map {
   void* g1 = nullptr;      // Ny x W complex scratch (derivative column output)
   void* ntab = nullptr;    // node profiles and taps of the last mlc_map_convolve_nodes call
   size_t ntab_bytes = 0;
+  mlc_ctx::ConstTable jobs; // device copy of the node jobs of that call
 };
 
 namespace mlc {

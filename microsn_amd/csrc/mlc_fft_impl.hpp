@@ -1367,7 +1367,7 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
   int itp = ws.add(sizeof(cpx<T>) * std::max<int64_t>(nt, 1));
   ws.commit();
   const NodeJob* dj =
-      static_cast<const NodeJob*>(const_table(ctx, kTableNodeJobs, jobs.data(), sizeof(NodeJob) * cnt));
+      static_cast<const NodeJob*>(const_table_in(ctx, m->jobs, jobs.data(), sizeof(NodeJob) * cnt));
   double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
   double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
   cpx<T>* tp = ws.get<cpx<T>>(itp);

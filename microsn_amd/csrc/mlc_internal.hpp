@@ -218,6 +218,10 @@ int fill_grid(mlc_ctx* ctx, int per_cu);
 // into a graph once it has run).
 enum ConstSlot { kTableBins = 0, kTableNodeJobs = 1, kTableBatchJobs = 2, kTableBatchMaps = 3 };
 const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes);
+// The same for a table owned by another object (e.g. a map's node jobs, so
+// that several maps used in one step each keep theirs on the device).
+const void* const_table_in(mlc_ctx* ctx, mlc_ctx::ConstTable& t, const void* host, size_t bytes);
+void const_table_free(mlc_ctx::ConstTable& t);
 
 // Workgroups of `threads` lanes of `kernel` resident at once on one CU (the
 // occupancy limit of its registers and LDS), at least 1; cached per kernel.

@@ -9,11 +9,17 @@ own shapes as parity cases:
              w in 0.05..0.5 px/day), 16384 trajectories: the FP32 adjoint
              against the FP64 adjoint, the FP64 adjoint against central
              differences (h = 1e-3)
+  config 3   64 maps of 1024^2..4096^2 (16 systems x 4 images), 20..400
+             epochs per curve, all curves in one batched call; the whole
+             step captured as one HIP graph
   config 4   10000^2 FP32 maps of 4 images x 5 bands with disc profiles of
              1..20 px, 1e6 trajectories per image, 200 epochs, 1000-bin table
-             over all 4e6 chi2: size-independent properties (FP32 against
-             FP64 on a sample, sortedness and stable order against numpy, bin
-             table against a host search of the sorted keys)
+             over all 4e6 chi2, the whole step captured as one HIP graph:
+             replays bit-equal to eager, then size-independent properties
+             (FP32 against FP64 on a sample, sortedness and stable order
+             against numpy, bin table against a host search of the sorted
+             keys)
+The workload builders are bench.py's (config3_workload, config4_workload).
 """
 import numpy as np
 import pytest
@@ -155,110 +161,101 @@ def test_config2_differentiation(ctx):
     assert np.quantile(err, 0.9) < 2e-2, np.quantile(err, 0.9, axis=1)
 
 
-def test_config4_scale_properties(ctx):
-    N, ntraj, nimg, nb, E = 10000, 1_000_000, 4, 5, 40      # 40 epochs x 5 bands = 200
-    radii = [1.0, 2.5, 5.0, 10.0, 20.0]
-    rng = np.random.default_rng(4004)
-    T = 100.0
-    times = [np.sort(rng.uniform(0, T, E)) for _ in range(nb)]
-    sets = [[(tb, rng.normal(-1.7, 0.02, E), np.full(E, 0.02)) for tb in times]]
-    obs = lcm.Observations(ctx, nb, sets)
-    prm = _trajectories(rng, ntraj, N, 200.0, T, 230.0)
-    prm[:, 4] = 1.0
-    P = torch.tensor(prm, device="cuda")
-    pr = torch.zeros(ntraj, dtype=torch.int32, device="cuda")
-    chis = []
-    for img in range(nimg):
-        m = _lognormal_map(ctx, N, 4100 + img)
-        spec = api.MapSpectrum(ctx, m, "f32")
-        nodes = lcm.NodeSet(ctx, spec, ["uniform"], [[[r] for r in radii]], radii)
-        print("config 4 image", img, flush=True)
+def test_config4_scale_captured(ctx):
+    """Four 10000^2 FP32 maps (one per image) x 5 bands of discs 1..20 px,
+    1e6 trajectories per image, 200 epochs: map spectra, 20 node maps, the
+    batched chi2 + adjoint of all 4e6 curves, their stable sort and a
+    1000-bin table captured as one HIP graph; replays equal the eager step
+    bit for bit.  Then FP32 against FP64 on a sample, the device sort against
+    numpy's stable argsort and the bin table against a host search."""
+    import bench
+
+    w = bench.config4_workload(ctx)
+    print("config 4 eager step", flush=True)
+    eager = [x.clone() for x in bench.batch_step(ctx, w)]
+    torch.cuda.synchronize()
+    print("config 4 capture", flush=True)
+    g, out, gen = bench.capture(ctx, lambda: bench.batch_step(ctx, w))
+    for r in range(2):
+        assert ctx.generation() == gen
+        for x in out:
+            x.zero_()
+        g.replay()
         torch.cuda.synchronize()
-        c, g = lcm.chi2_grad(ctx, nodes, obs, P, pr, pr)
-        torch.cuda.synchronize()
-        assert bool(torch.isfinite(c).all()) and bool(torch.isfinite(g).all())
-        chis.append(c)
-        if img == 0:
-            # FP32 against FP64 on a sample of 20000 curves (FP64 node maps)
-            s64 = api.MapSpectrum(ctx, m, "f64")
-            n64 = lcm.NodeSet(ctx, s64, ["uniform"], [[[r] for r in radii]], radii)
-            idx = torch.tensor(rng.choice(ntraj, 20000, replace=False), device="cuda")
-            c64, _ = lcm.chi2_grad(ctx, n64, obs, P[idx].contiguous(), pr[idx].contiguous(),
-                                   pr[idx].contiguous())
-            rel = ((c[idx] - c64).abs() / c64.abs()).max()
-            assert float(rel) <= 1e-5, float(rel)
-            del n64, s64
-        del nodes, spec, m
-        torch.cuda.empty_cache()
-    allc = torch.cat(chis)
-    n = allc.numel()
-    print("config 4 sort", n, flush=True)
-    perm, _, sk = api.sort_f64(ctx, allc, sorted_keys=True)
-    host = allc.cpu().numpy()
+        for a, b in zip(out, eager):
+            assert torch.equal(a, b), r
+    c, gr, perm, sk = eager[:4]
+    assert bool(torch.isfinite(c).all()) and bool(torch.isfinite(gr).all())
+    # FP32 against FP64 on 20000 curves of image 0 (FP64 node maps)
+    ntraj, radii = w["ntraj"], w["radii"]
+    s64 = api.MapSpectrum(ctx, w["maps"][0], "f64")
+    n64 = lcm.NodeSet(ctx, s64, ["uniform"], [[[r] for r in radii]], radii)
+    rng = np.random.default_rng(4005)
+    idx = torch.tensor(rng.choice(ntraj, 20000, replace=False), device="cuda")
+    z = torch.zeros(20000, dtype=torch.int32, device="cuda")
+    c64, _ = lcm.chi2_grad(ctx, n64, w["obs"], w["params"][idx].contiguous(), z, z)
+    rel = ((c[idx] - c64).abs() / c64.abs()).max()
+    assert float(rel) <= 1e-5, float(rel)
+    del n64
+    s64.close()
+    host = c.cpu().numpy()
+    print("config 4 sort", host.size, flush=True)
     want = np.argsort(host, kind="stable")
     assert np.array_equal(perm.cpu().numpy().astype(np.int64), want)
     sk_h = sk.cpu().numpy()
     assert np.array_equal(sk_h, host[want])
-    bins = api.mpd_bins(1000, 1.0, 1e7)
-    lo, npb = api.bin_edges(ctx, sk, bins)
-    up = np.searchsorted(sk_h, bins, side="right")
+    lo, npb, bc = (eager[4].cpu().numpy(), eager[5].cpu().numpy(), eager[6].cpu().numpy())
+    up = np.searchsorted(sk_h, w["bins"], side="right")
     assert np.array_equal(np.concatenate([[0], up[:-1]]), lo.astype(np.int64))
     assert np.array_equal(np.diff(np.concatenate([[0], up])), npb.astype(np.int64))
-    sb = api.Chi2SortBins(ctx, 1, bins.size)
-    sb.lower_ind[:], sb.n_per_bin[:] = lo, npb
-    bc, be = api.bin_chi2_GPU(ctx, sb, sk)
-    for i in np.nonzero(npb)[0][:: 50]:
+    for i in np.nonzero(npb)[0][::50]:
         seg = sk_h[lo[i]: lo[i] + npb[i]]
         assert abs(bc[i] - seg.mean()) <= 1e-12 * abs(seg.mean())
+    for sp in w["specs"]:
+        sp.close()
 
 
-def test_config3_batching_properties(ctx):
+def test_config3_batching_one_call(ctx):
     """64 maps of 1024^2..4096^2 with kappa, gamma in 0.2..0.6 ((1-k)^2-g^2 >
     0.05), 16 systems of 4 images with flux ratios log-uniform in 0.1..1 (the
-    per-image magnitude offset), light curves of 20..400 epochs: every map's
-    FP32 chi2 within 1e-5 of FP64, and the pooled bin table identical."""
-    rng = np.random.default_rng(3003)
-    sizes = [1024, 1280, 1536, 2048, 2560, 3072, 4096]
-    nsys, nimg, ncur = 16, 4, 2048
-    nsets = 64
-    sets = []
-    for _ in range(nsets):
-        ne = int(rng.integers(20, 401))
-        t = np.sort(rng.uniform(0, 100.0, ne))
-        sets.append([(t, rng.normal(-1.0, 0.3, ne), np.full(ne, 0.05))])
-    obs = lcm.Observations(ctx, 1, sets)
-    all32, all64 = [], []
-    for s in range(nsys):
-        ratios = np.exp(rng.uniform(np.log(0.1), np.log(1.0), nimg))
-        for i in range(nimg):
-            N = int(rng.choice(sizes))
-            while True:
-                k, g = rng.uniform(0.2, 0.6, 2)
-                if (1 - k) ** 2 - g ** 2 > 0.05:
-                    break
-            mean = 1.0 / abs((1 - k) ** 2 - g ** 2)
-            m = _lognormal_map(ctx, N, 3100 + s * nimg + i, mean=mean)
-            prm = _trajectories(rng, ncur, N, 200.0, 100.0, 230.0)
-            prm[:, 4] = 2.0
-            prm[:, 6] = -2.5 * np.log10(ratios[i])
-            P = torch.tensor(prm, device="cuda")
-            pr = torch.zeros(ncur, dtype=torch.int32, device="cuda")
-            oi = torch.tensor(rng.integers(0, nsets, ncur).astype(np.int32), device="cuda")
-            res = {}
-            for prec in ("f32", "f64"):
-                spec = api.MapSpectrum(ctx, m, prec)
-                nodes = lcm.NodeSet(ctx, spec, ["gaussian"], [[[2.0]]], [1.0])
-                res[prec], _ = lcm.chi2_grad(ctx, nodes, obs, P, pr, oi)
-                del nodes
-                spec.close()
-            rel = ((res["f32"] - res["f64"]).abs() / res["f64"].abs()).max()
-            assert float(rel) <= 1e-5, (s, i, N, float(rel))
-            all32.append(res["f32"])
-            all64.append(res["f64"])
-    print("config 3 maps", nsys * nimg, flush=True)
-    bins = api.mpd_bins(100, 1.0, 1e7)
-    tables = []
-    for allc in (torch.cat(all32), torch.cat(all64)):
-        _, _, sk = api.sort_f64(ctx, allc, sorted_keys=True)
-        tables.append(api.bin_edges(ctx, sk, bins))
-    assert np.array_equal(tables[0][0], tables[1][0]) and np.array_equal(tables[0][1], tables[1][1])
+    per-image magnitude offset), light curves of 20..400 epochs, all curves
+    of all maps in one batched call: FP32 within 1e-5 of FP64 on every curve,
+    the pooled bin tables identical, every map's curves equal to a per-map
+    call bit for bit at the same launch shape, and the whole step (64 map
+    spectra, 64 node maps, the batched call, sort and bins) captured as one
+    HIP graph whose replays equal the eager step."""
+    import bench
+
+    ctx.tune_force(16, 1)
+    try:
+        res = {}
+        for prec in ("f32", "f64"):
+            w = bench.config3_workload(ctx, prec)
+            print("config 3", prec, "maps", len(w["maps"]), flush=True)
+            res[prec] = (w, [x.clone() for x in bench.batch_step(ctx, w)])
+        w, out32 = res["f32"]
+        _, out64 = res["f64"]
+        c32, c64 = out32[0], out64[0]
+        rel = ((c32 - c64).abs() / c64.abs()).max()
+        assert float(rel) <= 1e-5, float(rel)
+        assert torch.equal(out32[4], out64[4]) and torch.equal(out32[5], out64[5])
+        midx = w["map_idx"].cpu().numpy()
+        for m, ns in enumerate(w["nodesets"]):
+            sel = torch.tensor(np.nonzero(midx == m)[0], device="cuda")
+            rc, rg = lcm.chi2_grad(ctx, ns, w["obs"], w["params"][sel].contiguous(), w["prof"][sel].contiguous(),
+                                   w["obs_idx"][sel].contiguous())
+            assert torch.equal(rc, c32[sel]) and torch.equal(rg, out32[1][sel]), m
+        g, out, gen = bench.capture(ctx, lambda: bench.batch_step(ctx, w))
+        for r in range(2):
+            assert ctx.generation() == gen
+            for x in out:
+                x.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            for a, b in zip(out, out32):
+                assert torch.equal(a, b), r
+        for prec in res:
+            for sp in res[prec][0]["specs"]:
+                sp.close()
+    finally:
+        ctx.tune_mode(ctx.TUNE_AUTO)
