@@ -347,6 +347,58 @@ def capture(ctx, fn):
     return g, out, ctx.generation()
 
 
+def time_batch_workload(ctx, which, steps):
+    """The batching or scale workload: one eager step (per-kernel events),
+    then the step captured as one HIP graph and its replays timed; the
+    batched band kernel's roofline from its event-timed launch."""
+    import torch
+
+    w = config3_workload(ctx) if which == "config3" else config4_workload(ctx)
+    torch.cuda.synchronize()
+    batch_step(ctx, w)  # first step: launch shapes tuned, tables and workspace in place
+    torch.cuda.synchronize()
+    ctx.timing_reset()
+    ctx.timing(True)
+    eager = batch_step(ctx, w)
+    torch.cuda.synchronize()
+    ctx.timing(False)
+    kern = ctx.timing_all()
+    g, out, gen = capture(ctx, lambda: batch_step(ctx, w))
+    same = False
+    g.replay()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(out, eager))
+    if ctx.generation() != gen:
+        raise RuntimeError("the captured step refers to memory the library has since replaced")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ncurves = int(w["params"].shape[0])
+    off = w["obs"].off.cpu().numpy()
+    oi = w["obs_idx"].cpu().numpy()
+    B = w["obs"].B
+    samples = int((off[oi * B + B] - off[oi * B]).sum())
+    ms, launches = kern["lc_batch"]
+    ab = samples * 4 * 8 + ncurves * (64 + 64)  # FP32 (mu, dmu/dr) taps; record read, accumulator written
+    dom = max(kern, key=lambda k: kern[k][0])
+    rec = {"ms_per_step": dt * 1e3, "light_curves_per_s": ncurves / dt, "curves": ncurves,
+           "samples": samples, "maps": len(w["maps"]),
+           "map_sizes": sorted({int(m.shape[0]) for m in w["maps"]}), "graph_replay_matches_eager": same,
+           "steps": steps, "dominant_kernel": dom,
+           "kernels_ms_per_step": {k: v[0] for k, v in sorted(kern.items())},
+           "lc_batch_roofline": {"bound": "hbm", "achieved": ab / (ms / launches / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
+                                 "unit": "GB/s", "frac": ab / (ms / launches / 1e3) / HBM_PEAK,
+                                 "algorithmic_bytes_per_launch": ab, "avg_launch_ms": ms / launches}}
+    for sp in w["specs"]:
+        sp.close()
+    del w, eager, out, g
+    torch.cuda.empty_cache()
+    return rec
+
+
 def cpu_model():
     """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
     try:
@@ -430,6 +482,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batching (64 maps) and scale (10000^2) workloads")
     ap.add_argument("--eager", action="store_true",
                     help="time the FP32 step as eager launches instead of replays of its captured graph")
     ap.add_argument("--pmc-step", action="store_true",
@@ -540,9 +594,16 @@ def main():
     roof = roofline(dom)
     roof_all = {k: roofline(k) for k in sorted(timed, key=lambda k: -kern[k][0])}
 
+    extras = None
+    if not args.no_extras:
+        del ws32
+        torch.cuda.empty_cache()
+        extras = {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
+                  "config4_scale_10000": time_batch_workload(ctx, "config4", max(3, args.steps // 4))}
+
     cpu = None
     if not args.no_cpu:
-        cpu, chi_host = cpu_baseline(m, params, prof, times, mobs, sig, ws32["bins"])
+        cpu, chi_host = cpu_baseline(m, params, prof, times, mobs, sig, api.mpd_bins(NBINS, 1.0, 1.0e7, True))
         fin = np.isfinite(chi_host)
         err["chi2_max_rel_err_vs_host_oracle"] = float(
             np.max(np.abs(chi32.cpu().numpy()[fin] - chi_host[fin]) / np.abs(chi_host[fin])))
@@ -562,6 +623,7 @@ def main():
         "ms_per_step_with_kernel_events": dt32_ev * 1e3,
         "hip_graph": graph_info,
         "setup_s": t_setup,
+        "extras": extras,
     }
     print(json.dumps(line))
 
