@@ -673,16 +673,25 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
   }
   if (t < R) bufs[t * Gm::LDS + pad16(N / 2)] = mk<T>(0, 0);  // Nyquist column
   __syncthreads();
-  const int g = t / G, tl = t % G, y = y0 + g;
-  cpx<T>* buf = bufs + g * Gm::LDS;
-  ct_fft<T, N, true>(buf, lo, hi, tl);
-  if (y >= Ny) return;  // after the last barrier
+  const int g = t / G, tl = t % G;
+  ct_fft<T, N, true>(bufs + g * Gm::LDS, lo, hi, tl);  // ends with a barrier: all R rows done
+  // Stores of whole pixel pairs: chunk s is pixels (x, x+1), x = 4 (s / 2R)
+  // + 2 (s % 2), of row y0 + (s % 2R) / 2, one 16-byte store; the 2R chunks
+  // of one tile are consecutive lanes and fill the workgroup's R rows of that
+  // 4 x 4 tile with contiguous bytes (64 bytes for R = 2, the whole 128-byte
+  // tile for R = 4), instead of 32-byte pieces per row.
   const T sh = shift ? (T)*shift : (T)0;
+  typedef T v4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int x = tl + j * G;
-    const cpx<T> z = buf[pad16(x)];
-    out[tile_offset(x, y, N)] = mk<T>(z.x + sh, z.y);
+  for (int i = 0; i < N / (2 * G); ++i) {
+    const int sidx = t + i * NT, tile = sidx / (2 * R), part = sidx % (2 * R);
+    const int row = part >> 1, x = 4 * tile + 2 * (part & 1), y = y0 + row;
+    const cpx<T>* b = bufs + row * Gm::LDS;
+    const cpx<T> z0 = b[pad16(x)], z1 = b[pad16(x + 1)];
+    if (y < Ny) {
+      const v4 val = {z0.x + sh, z0.y, z1.x + sh, z1.y};
+      *reinterpret_cast<v4*>(out + tile_offset(x, y, N)) = val;
+    }
   }
 }
 
