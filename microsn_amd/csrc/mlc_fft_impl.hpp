@@ -85,6 +85,45 @@ __device__ void dev_fft(cpx<T>* buf, const Plan& plan, const TW& tw, PAD pad, in
   }
 }
 
+// The same transform with the plan fixed at compile time (length L known to
+// the compiler): the stages are those make_plan(L, 8) gives, in the same
+// order with the same arithmetic, so results are bit-identical to dev_fft,
+// but every index division and modulo by the stage constants folds into
+// multiplies and shifts.
+struct FixedPlan {
+  int n;
+  int r[24];
+};
+
+constexpr FixedPlan fixed_plan(int L) {
+  FixedPlan p{0, {}};
+  int m = L;
+  while (m % 8 == 0) { p.r[p.n++] = 8; m /= 8; }
+  while (m % 4 == 0) { p.r[p.n++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.r[p.n++] = 2; m /= 2; }
+  while (m % 5 == 0) { p.r[p.n++] = 5; m /= 5; }
+  while (m % 3 == 0) { p.r[p.n++] = 3; m /= 3; }
+  return p;
+}
+
+template <class T, int L, int G, int EPT, bool INV, int S, int NS, class TW, class PAD>
+__device__ __forceinline__ void fixed_fft(cpx<T>* buf, const TW& tw, PAD pad, int g) {
+  constexpr FixedPlan P = fixed_plan(L);
+  if constexpr (S < P.n) {
+    constexpr int R = P.r[S];
+    dev_stage<T, R, G, EPT, INV>(buf, L, NS, tw, pad, g);
+    fixed_fft<T, L, G, EPT, INV, S + 1, NS * R>(buf, tw, pad, g);
+  }
+}
+
+// Transform of a row of runtime length (ML == 0, plan `plan`) or of the
+// compile-time length ML.
+template <class T, int ML, int G, bool INV, class TW, class PAD>
+__device__ __forceinline__ void row_fft(cpx<T>* buf, const Plan& plan, const TW& tw, PAD pad, int g) {
+  if constexpr (ML > 0) fixed_fft<T, ML, G, 16, INV, 0, 1>(buf, tw, pad, g);
+  else dev_fft<T, G, 16, INV>(buf, plan, tw, pad, g);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; this numbering gives
 // each XCD a contiguous range of logical blocks.  The grid is launched with
 // 8*ceil(nb/8) workgroups; ids >= nb return at once.
@@ -106,14 +145,16 @@ __device__ __forceinline__ int wrap_row(int y, int h, int N) {
 // With shift != NULL the FP64 value *shift is subtracted before rounding to T.
 // With rowsum != NULL the row's own FP64 mean is subtracted instead and its
 // FP64 sum goes to rowsum[y] (k_row_dc_fix then sets the DC terms).
-template <class T, class TIN, int G>
+// ML > 0: the half length M = Nx/2 is the compile-time constant ML.
+template <class T, class TIN, int G, int ML>
 __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T>* __restrict__ out,
                                                int ldo, int Nx, const double* __restrict__ shift,
                                                double* __restrict__ rowsum, Plan plan,
                                                const cpx<T>* __restrict__ twM,
                                                const cpx<T>* __restrict__ twN) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int y = blockIdx.x, g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
+  const int y = blockIdx.x, g = threadIdx.x, M = ML > 0 ? ML : Nx >> 1, QL = tw_len(M);
+  if constexpr (ML > 0) Nx = 2 * ML;
   cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* buf = q + QL;
   const NoPad P;
@@ -152,7 +193,7 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
     if (n < M) buf[P(n)] = mk<T>((T)((double)v[i].x - sh), (T)((double)v[i].y - sh));
   }
   __syncthreads();
-  dev_fft<T, G, 16, false>(buf, plan, QuarterTw<T>{q, QL}, P, g);
+  row_fft<T, ML, G, false>(buf, plan, QuarterTw<T>{q, QL}, P, g);
   cpx<T>* o = out + (size_t)y * ldo;
   for (int k = g; k <= (M >> 1); k += G) {
     cpx<T> Xk, Xm;
@@ -204,7 +245,8 @@ __device__ __forceinline__ void c2r_fill(cpx<T>* buf, const cpx<T>* __restrict__
 // the four rows of a tile are written through one L2.  MODE 3 / MODE 4:
 // only the first (x0 + *shift) / second (x1) member of every node pair, from
 // in0 alone, for rows too long to hold two buffers in LDS.
-template <class T, int G, int MODE>
+// ML > 0: the half length M = Nx/2 is the compile-time constant ML.
+template <class T, int G, int MODE, int ML>
 __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
                                                const cpx<T>* __restrict__ in1, T* __restrict__ out,
                                                int Nx, int Ny, const double* __restrict__ shift,
@@ -213,6 +255,7 @@ __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
   extern __shared__ __align__(16) unsigned char smem[];
   const int y = MODE >= 2 ? xcd_block(Ny) : (int)blockIdx.x;
   if (y >= Ny) return;
+  if constexpr (ML > 0) Nx = 2 * ML;
   const int g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
   cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* buf0 = q + QL;
@@ -223,17 +266,18 @@ __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
   if constexpr (MODE == 2) c2r_fill<T, G>(buf1, in1 + (size_t)y * W, M, twN, g);
   __syncthreads();
   const QuarterTw<T> tw{q, QL};
-  dev_fft<T, G, 16, true>(buf0, plan, tw, P, g);
+  row_fft<T, ML, G, true>(buf0, plan, tw, P, g);
   const T sh = (MODE != 0 && shift) ? (T)*shift : (T)0;
   if constexpr (MODE == 2) {
-    dev_fft<T, G, 16, true>(buf1, plan, tw, P, g);
-    // (value, deriv) pairs of pixels 2n and 2n+1: adjacent in their tile row
+    row_fft<T, ML, G, true>(buf1, plan, tw, P, g);
+    // (value, deriv) pairs of pixels 2n and 2n+1: adjacent in their tile
+    // row, one 2-pair store
+    typedef T v4 __attribute__((ext_vector_type(4)));
     cpx<T>* o = reinterpret_cast<cpx<T>*>(out);
     for (int n = g; n < M; n += G) {
       cpx<T> a = buf0[P(n)], b = buf1[P(n)];
-      cpx<T>* dst = o + tile_offset(2 * n, y, Nx);
-      dst[0] = mk<T>(a.x + sh, b.x);
-      dst[1] = mk<T>(a.y + sh, b.y);
+      const v4 val = {a.x + sh, b.x, a.y + sh, b.y};
+      *reinterpret_cast<v4*>(o + tile_offset(2 * n, y, Nx)) = val;
     }
   } else if constexpr (MODE == 3 || MODE == 4) {
     cpx<T>* o = reinterpret_cast<cpx<T>*>(out);
@@ -979,6 +1023,26 @@ size_t row_lds(int M, int nbuf) {
   return sizeof(cpx<T>) * ((size_t)tw_len(M) + (size_t)nbuf * M);
 }
 
+// Calls f(integral_constant<int, M>) for the half lengths with a
+// compile-time row kernel (the widths 1000, 1280, 1536, 2000, 2560, 3072,
+// 5000 and 10000 of the workloads and of the reference's maps), else
+// f(integral_constant<int, 0>) (runtime plan).
+template <class F>
+void with_fixed_len(int M, F&& f) {
+  using std::integral_constant;
+  switch (M) {
+    case 500: f(integral_constant<int, 500>{}); break;
+    case 640: f(integral_constant<int, 640>{}); break;
+    case 768: f(integral_constant<int, 768>{}); break;
+    case 1000: f(integral_constant<int, 1000>{}); break;
+    case 1280: f(integral_constant<int, 1280>{}); break;
+    case 1536: f(integral_constant<int, 1536>{}); break;
+    case 2500: f(integral_constant<int, 2500>{}); break;
+    case 5000: f(integral_constant<int, 5000>{}); break;
+    default: f(integral_constant<int, 0>{}); break;
+  }
+}
+
 template <class T, class TIN>
 void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const double* shift,
              double* rowsum = nullptr, int ldo = 0) {
@@ -991,9 +1055,15 @@ void row_r2c(mlc_ctx* ctx, const TIN* in, cpx<T>* out, int Nx, int Ny, const dou
   if (lds > kLdsBytes) fail(MLC_ERR_UNSUPPORTED, "row FFT: row does not fit in LDS");
   with_row_group(M, "row FFT", [&](auto gc) {
     constexpr int G = decltype(gc)::value;
-    auto k = k_row_r2c<T, TIN, G>;
-    allow_lds(k, lds);
-    MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, ldo, Nx, shift, rowsum, p, twM, twN);
+    with_fixed_len(M, [&](auto lc) {
+      constexpr int ML = decltype(lc)::value;
+      if constexpr (ML == 0 || ML <= G * 16) {
+        auto k = k_row_r2c<T, TIN, G, ML>;
+        allow_lds(k, lds);
+        MLC_LAUNCH(ctx, "fft_row_r2c", k, dim3(Ny), dim3(G), lds, in, out, ldo, Nx, shift, rowsum, p, twM,
+                   twN);
+      }
+    });
   });
 }
 
@@ -1015,11 +1085,16 @@ void row_c2r(mlc_ctx* ctx, const cpx<T>* in0, const cpx<T>* in1, T* out, int Nx,
   if (lds > kLdsBytes) fail(MLC_ERR_UNSUPPORTED, "row FFT: row does not fit in LDS");
   with_row_group(M, "row FFT", [&](auto gc) {
     constexpr int G = decltype(gc)::value;
-    auto k = k_row_c2r<T, G, MODE>;
-    allow_lds(k, lds);
-    const int grid = MODE >= 2 ? ((Ny + 7) / 8) * 8 : Ny;
-    MLC_LAUNCH(ctx, MODE >= 2 ? "fft_row_c2r_node" : "fft_row_c2r", k, dim3(grid), dim3(G), lds, in0,
-               in1, out, Nx, Ny, shift, p, twM, twN);
+    with_fixed_len(M, [&](auto lc) {
+      constexpr int ML = decltype(lc)::value;
+      if constexpr (ML == 0 || ML <= G * 16) {
+        auto k = k_row_c2r<T, G, MODE, ML>;
+        allow_lds(k, lds);
+        const int grid = MODE >= 2 ? ((Ny + 7) / 8) * 8 : Ny;
+        MLC_LAUNCH(ctx, MODE >= 2 ? "fft_row_c2r_node" : "fft_row_c2r", k, dim3(grid), dim3(G), lds, in0,
+                   in1, out, Nx, Ny, shift, p, twM, twN);
+      }
+    });
   });
 }
 

@@ -12,7 +12,9 @@ from microsn_amd import api
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(8, 2), (16, 16), (48, 64), (60, 100), (250, 256), (1000, 1000), (96, 4096), (4096, 40)]
+SIZES = [(8, 2), (16, 16), (48, 64), (60, 100), (250, 256), (1000, 1000), (96, 4096), (4096, 40),
+         # the widths with compile-time row kernels (2 M for M in 500 ... 5000)
+         (8, 1280), (6, 1536), (16, 2000), (12, 2560), (8, 3072), (10, 5000), (8, 10000)]
 
 
 def _rel(a, b):
@@ -219,3 +221,22 @@ def test_context_close_closes_its_maps():
     assert spec.h is None
     spec.close()
     del spec
+
+
+@pytest.mark.parametrize("Ny,Nx", [(40, 1000), (32, 1280), (24, 3072), (20, 10000)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_node_map_rows_at_fixed_widths(ctx, orc, Ny, Nx, prec):
+    """Node maps (mu, dmu/drhalf) at widths whose rows run the compile-time
+    row kernels: against the oracle's FP64 FFT convolutions of the profile
+    and of its derivative."""
+    rng = np.random.default_rng(Nx + Ny)
+    m = _smooth_map(rng, Ny, Nx)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
+    got = spec.convolve("gaussian", 3.0, value=False, node=True).double().cpu().numpy()
+    p, dp = orc.profile("gaussian", 3.0, 1.0, deriv=True)
+    want_v = orc.fft_convolve(m, orc.set_kernel(p, Nx, Ny))
+    want_d = orc.fft_convolve(m, orc.set_kernel(dp, Nx, Ny))
+    tol = 1e-12 if prec == "f64" else 2e-6
+    assert _rel(got[..., 0], want_v) < tol
+    assert _rel(got[..., 1], want_d) < tol * 10
+    spec.close()
