@@ -399,6 +399,69 @@ def time_batch_workload(ctx, which, steps):
     return rec
 
 
+def time_pair_mode(ctx, steps=3):
+    """The reference's pair mode at grid_input.json's shape (9801 locations
+    = 9.6e7 image pairs, 5 source sizes, 300 ratio bins, the 5-point observed
+    curve): setup_integral (ratios z, their stable sort and the rank of every
+    pair, bin table), the pair chi2 in sorted order and the binned means,
+    timed per call on synthetic light curves (log-normal point samples, the
+    convolved columns a smooth function of size)."""
+    import torch
+
+    import microsn_amd.api as api
+
+    nloc, sizes = 9801, np.array([0.0, 0.05, 0.1, 0.2, 0.3])
+    rng = np.random.default_rng(5005)
+    lcs = []
+    for _ in range(2):
+        mu0 = np.exp(rng.normal(1.5, 0.6, nloc))
+        grow = 1.0 + rng.normal(0.0, 0.05, (nloc, 1)) * sizes[None, 1:] / 0.3
+        lc = api.SimLC(ctx, nloc, sizes.size)
+        cols = np.concatenate([mu0[:, None], mu0[:, None] * grow], 1)
+        lc.LC.copy_(torch.tensor(cols, device="cuda"))
+        lc.DLC.copy_(torch.tensor(np.diff(cols, axis=1), device="cuda"))
+        lcs.append(lc)
+    LCA, LCB = lcs
+    bins = api.mpd_bins(300, 0.0001, 300.0)
+    sb = api.Chi2SortBins(ctx, nloc, bins.size)
+    Dfac = np.sqrt(1750.0 * 850.0 / 1570.0)
+    cv = api.setup_chi2_calculation((0.4, 0.5), (1.0,), Dfac, [0.7, 1.2, 1.4, 1.1, 0.9], [5, 7, 12, 14, 17],
+                                    [0.1, 0.15, 0.1, 0.1, 0.15], sizes)
+    chi2 = torch.empty(nloc * nloc, dtype=torch.float64, device="cuda")
+    t = {"setup_integral": 0.0, "calculate_chi2": 0.0, "bin_chi2": 0.0}
+    for it in range(steps + 1):  # the first round settles the workspace
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        api.setup_integral_GPU(ctx, LCA, LCB, bins, sb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        api.calculate_chi2_GPU(ctx, cv, chi2, sb, LCA, LCB)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        api.bin_chi2_GPU(ctx, sb, chi2)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if it > 0:
+            t["setup_integral"] += (t1 - t0) / steps
+            t["calculate_chi2"] += (t2 - t1) / steps
+            t["bin_chi2"] += (t3 - t2) / steps
+    ctx.timing_reset()
+    ctx.timing(True)
+    api.setup_integral_GPU(ctx, LCA, LCB, bins, sb)
+    api.calculate_chi2_GPU(ctx, cv, chi2, sb, LCA, LCB)
+    api.bin_chi2_GPU(ctx, sb, chi2)
+    torch.cuda.synchronize()
+    ctx.timing(False)
+    kern = ctx.timing_all()
+    total = sum(t.values())
+    rec = {"pairs": nloc * nloc, "locations": nloc, "bins": int(bins.size),
+           "ms": {k: v * 1e3 for k, v in t.items()}, "ms_total": total * 1e3, "pairs_per_s": nloc * nloc / total,
+           "kernels_ms": {k: v[0] for k, v in sorted(kern.items())}}
+    del chi2, sb, LCA, LCB, lcs
+    torch.cuda.empty_cache()
+    return rec
+
+
 def cpu_model():
     """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
     try:
@@ -599,7 +662,8 @@ def main():
         del ws32
         torch.cuda.empty_cache()
         extras = {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
-                  "config4_scale_10000": time_batch_workload(ctx, "config4", max(3, args.steps // 4))}
+                  "config4_scale_10000": time_batch_workload(ctx, "config4", max(3, args.steps // 4)),
+                  "pair_mode_grid_input": time_pair_mode(ctx)}
 
     cpu = None
     if not args.no_cpu:
