@@ -547,6 +547,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batching (64 maps) and scale (10000^2) workloads")
+    ap.add_argument("--extras-only", action="store_true",
+                    help="run only the batching, scale and pair-mode workloads (for profiling them)")
     ap.add_argument("--eager", action="store_true",
                     help="time the FP32 step as eager launches instead of replays of its captured graph")
     ap.add_argument("--pmc-step", action="store_true",
@@ -565,6 +567,13 @@ def main():
 
     if args.pmc_step:
         pmc_step()
+        return
+    if args.extras_only:
+        ctx = api.Context(0)
+        print(json.dumps({"extras": {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
+                                     "config4_scale_10000": time_batch_workload(ctx, "config4",
+                                                                                max(3, args.steps // 4)),
+                                     "pair_mode_grid_input": time_pair_mode(ctx)}}))
         return
     t_setup = time.perf_counter()
     m = make_map()
