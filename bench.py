@@ -96,6 +96,16 @@ def algorithmic_bytes(name, esz, ncurves):
         "fft_row_c2r_node": 2 * spec + 2 * real,       # read 2 spectra, write (mu, dmu/dr)
         "fft_row_r2c": N * N * 8 + spec,               # read FP64 map, write spectrum
         "lc_chi2_grad": curves * EPOCHS * 4 * 2 * esz + curves * (64 + 2 * 64),
+        # the FP64 plain path's stages (esz = 8)
+        "fft_col_fwd": 2 * spec,                       # read + write the spectrum, in place
+        "fft_col_inv": 2 * spec,
+        "fft_row_c2r": spec + real,                    # read spectrum, write real map
+        "plain_make_kernel": real,                     # write the full-size kernel array
+        "plain_cmul": 3 * spec,                        # read two spectra, write one
+        "plain_interleave": 4 * real,                  # read value + derivative, write the pairs
+        # per (curve, epoch) of all bands: 4 taps of (mu, dmu/dr), write 4 FP64 terms
+        "lc_plain_samples": ncurves * EPOCHS * len(BAND_SCALE) * (4 * 2 * esz + 32),
+        "lc_plain_reduce": ncurves * EPOCHS * len(BAND_SCALE) * (32 + 24) + ncurves * 64,
     }.get(name)
 
 
@@ -462,6 +472,148 @@ def time_pair_mode(ctx, steps=3):
     return rec
 
 
+def replay_ms(ctx, fn, steps):
+    """fn() captured as one HIP graph (after an eager run that settles the
+    launch shapes and tables): ms per replay over `steps` replays, and
+    whether a replay reproduces the eager outputs bit for bit."""
+    import torch
+
+    eager = [x.clone() for x in fn()]
+    torch.cuda.synchronize()
+    g, out, gen = capture(ctx, fn)
+    g.replay()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(out, eager))
+    if ctx.generation() != gen:
+        raise RuntimeError("the captured step refers to memory the library has since replaced")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, same
+
+
+def time_config0(ctx, steps, cpu=True):
+    """The CPU-runnable workload: 1000^2 map (log-normal, 3 px smoothing,
+    mean 5), Gaussian source of sigma 5 px, 1000 trajectories of 200 px over
+    100 days, 50 irregular epochs against one observed curve with 0.05 mag
+    noise, chi2 + adjoint, sort, 100 bins: GPU FP32 and FP64 fused (graph
+    replays), GPU FP64 plain (eager, reference-structured) and the FP64 host
+    oracle (OpenMP) on the same step, in light curves/s."""
+    import torch
+
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    n, ncur, E, T = 1000, 1000, 50, 100.0
+    rng = np.random.default_rng(1000)
+    m = lognormal_map(ctx, n, 1000)
+    rhalf = 5.0 * np.sqrt(2 * np.log(2))
+    t = np.sort(rng.uniform(0, T, E))
+    prm = trajectories(rng, ncur, n, 200.0, T, 230.0)
+    prm[:, 4] = rhalf
+    mobs = rng.normal(-1.75, 0.05, E)
+    sig = np.full(E, 0.05)
+    obs = lcm.Observations(ctx, 1, [[(t, mobs, sig)]])
+    P = torch.tensor(prm, device="cuda")
+    z = torch.zeros(ncur, dtype=torch.int32, device="cuda")
+    bins = api.mpd_bins(100, 1.0, 1e6)
+    rec = {"curves": ncur, "epochs": E, "map": n}
+    for prec in ("f32", "f64"):
+        spec = api.MapSpectrum(ctx, m, prec)
+        nodes = lcm.NodeSet(ctx, spec, ["gaussian"], [[[rhalf]]], [1.0], compute=False)
+
+        def step(spec=spec, nodes=nodes):
+            spec.update(m)
+            c, g = lcm.chi2_grad(ctx, nodes, obs, P, z, z, convolve=True)
+            _, _, sk = api.sort_f64(ctx, c, sorted_keys=True)
+            lo, npb, _, be = api.bin_table_device(ctx, sk, bins)
+            return c, g, lo, npb, be
+        ms, same = replay_ms(ctx, step, steps)
+        rec[f"gpu_{prec}_fused"] = {"ms_per_step": ms, "light_curves_per_s": ncur / ms * 1e3,
+                                    "graph_replay_matches_eager": same}
+        if prec == "f64":
+            def plain(spec=spec, nodes=nodes):
+                spec.update(m)
+                nodes.compute(plain=True)
+                c, g = lcm.chi2_grad(ctx, nodes, obs, P, z, z, plain=True)
+                _, _, sk = api.sort_f64(ctx, c, sorted_keys=True)
+                return api.bin_table_device(ctx, sk, bins)
+            plain()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                plain()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / steps * 1e3
+            rec["gpu_f64_plain"] = {"ms_per_step": ms, "light_curves_per_s": ncur / ms * 1e3}
+        spec.close()
+    if cpu:
+        from oracle import oracle as orc
+        mh = m.cpu().numpy()
+        t0 = time.perf_counter()
+        v, dv = orc.profile("gaussian", rhalf, 1.0, deriv=True)
+        maps = np.stack([orc.fft_convolve(mh, orc.set_kernel(v, n, n)),
+                         orc.fft_convolve(mh, orc.set_kernel(dv, n, n))], -1)[None]
+        zz = np.zeros(ncur, np.int32)
+        c, _ = orc.lc_chi2_grad(maps, 1, [0], [1], [rhalf], None, [1.0], prm, zz, zz, np.array([0, E], np.int32),
+                                t, mobs, sig)
+        sk = c[orc.stable_argsort(c)]
+        lo, npb = orc.bin_edges(sk, bins)
+        orc.bin_chi2(sk, lo, npb)
+        dt = time.perf_counter() - t0
+        rec["cpu_oracle_f64"] = {"ms_per_step": dt * 1e3, "light_curves_per_s": ncur / dt,
+                                 "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    return rec
+
+
+def time_config2(ctx, steps):
+    """The differentiation workload: 2048^2 map, expanding uniform-disc
+    photosphere r(t) = r0 + w t (r0 = 1 px, w uniform in 0.05..0.5 px/day,
+    12 radius nodes from 1 to 12.5 px), 16384 trajectories of 60 px over 20
+    days, 40 epochs: one step (map, 12 node maps, chi2 + 7-parameter
+    adjoint) in FP32 and FP64, and, for scale, the 13 chi2 evaluations that
+    central differences in the seven parameters would need (one evaluation
+    per side of six of them plus the centre)."""
+    import torch
+
+    import microsn_amd.api as api
+    import microsn_amd.lightcurves as lcm
+
+    n, ncur, E, T = 2048, 16384, 40, 20.0
+    rng = np.random.default_rng(2002)
+    m = lognormal_map(ctx, n, 2002)
+    radii = [1.0, 1.5, 2.0, 2.75, 3.5, 4.5, 5.5, 6.75, 8.0, 9.5, 11.0, 12.5]
+    t = np.sort(rng.uniform(0, T, E))
+    obs = lcm.Observations(ctx, 1, [[(t, rng.normal(-1.7, 0.05, E), np.full(E, 0.05))]])
+    prm = trajectories(rng, ncur, n, 60.0, T, 80.0)
+    prm[:, 4] = 1.0
+    prm[:, 5] = rng.uniform(0.05, 0.5, ncur)
+    P = torch.tensor(prm, device="cuda")
+    z = torch.zeros(ncur, dtype=torch.int32, device="cuda")
+    rec = {"curves": ncur, "epochs": E, "map": n, "nodes": len(radii)}
+    for prec in ("f32", "f64"):
+        spec = api.MapSpectrum(ctx, m, prec)
+        nodes = lcm.NodeSet(ctx, spec, ["uniform"], [[radii]], [1.0], compute=False)
+
+        def step(spec=spec, nodes=nodes):
+            spec.update(m)
+            return lcm.chi2_grad(ctx, nodes, obs, P, z, z, convolve=True)
+        ms, same = replay_ms(ctx, step, steps)
+
+        # the adjoint against central differences on the same node maps: one
+        # chi2 + gradient call, and the 13 chi2 calls differences would take
+        ms_one, _ = replay_ms(ctx, lambda nodes=nodes: lcm.chi2_grad(ctx, nodes, obs, P, z, z), steps)
+
+        def fd13(nodes=nodes):
+            return [lcm.chi2_grad(ctx, nodes, obs, P, z, z)[0] for _ in range(13)]
+        ms_fd, _ = replay_ms(ctx, fd13, steps)
+        rec[prec] = {"ms_per_step": ms, "light_curves_per_s": ncur / ms * 1e3, "graph_replay_matches_eager": same,
+                     "ms_chi2_and_adjoint_call": ms_one, "ms_13_chi2_calls": ms_fd}
+        spec.close()
+    return rec
+
+
 def cpu_model():
     """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
     try:
@@ -573,7 +725,9 @@ def main():
         print(json.dumps({"extras": {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
                                      "config4_scale_10000": time_batch_workload(ctx, "config4",
                                                                                 max(3, args.steps // 4)),
-                                     "pair_mode_grid_input": time_pair_mode(ctx)}}))
+                                     "pair_mode_grid_input": time_pair_mode(ctx),
+                                     "config0_1000": time_config0(ctx, args.steps, cpu=False),
+                                     "config2_photosphere_2048": time_config2(ctx, args.steps)}}))
         return
     t_setup = time.perf_counter()
     m = make_map()
@@ -675,7 +829,9 @@ def main():
         torch.cuda.empty_cache()
         extras = {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
                   "config4_scale_10000": time_batch_workload(ctx, "config4", max(3, args.steps // 4)),
-                  "pair_mode_grid_input": time_pair_mode(ctx)}
+                  "pair_mode_grid_input": time_pair_mode(ctx),
+                  "config0_1000": time_config0(ctx, args.steps, cpu=not args.no_cpu),
+                  "config2_photosphere_2048": time_config2(ctx, args.steps)}
 
     cpu = None
     if not args.no_cpu:
