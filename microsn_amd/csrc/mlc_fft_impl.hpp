@@ -739,6 +739,128 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
   }
 }
 
+// Node-map rows of a width N that is not a power of two (N = 2^a 3^b 5^c,
+// a compile-time constant): the y-filter of k_node_fused and one complex
+// inverse FFT of length N per row with the compile-time plan of fixed_fft
+// (radices 8, 4, 2, 5, 3; quarter-wave twiddle table in LDS), G threads per
+// row (a multiple of 64 with 16 G >= N), R rows per workgroup.  Column
+// pairs and store chunks that the thread count does not divide evenly are
+// predicated.  The filter's sums and their order are those of k_node_fused.
+struct Pad16 {
+  MLC_HD int operator()(int i) const { return pad16(i); }
+};
+
+template <class T, int N, int R, int G>
+__global__ __launch_bounds__(G * R) void k_node_fused_mix(
+    const cpx<T>* __restrict__ Gs, int ldg, const cpx<T>* __restrict__ tp, int ldt, int hq,
+    const double* __restrict__ phx, cpx<T>* __restrict__ out, int Ny, const double* __restrict__ shift,
+    const cpx<T>* __restrict__ twN) {
+  using P2 = CPair<T>;
+  constexpr int NT = G * R, QL = N / 4, LDSN = N + N / 16, NW = R + kNfK - 1;
+  static_assert(N % 4 == 0 && G % 64 == 0 && 16 * G >= N, "row geometry");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* qt = reinterpret_cast<cpx<T>*>(smem);  // W_N^m, m < N/4
+  cpx<T>* bufs = qt + QL;                         // R buffers of LDSN elements
+  const int nq = (Ny + R - 1) / R;
+  const int q = xcd_block(nq);
+  if (q >= nq) return;  // uniform per workgroup
+  const int y0 = q * R, t = threadIdx.x;
+  for (int i = t; i < QL; i += NT) qt[i] = twN[i];
+  const double sc = 0.5 / (double)N;
+  auto gpair = [&](int r, int u) -> P2 {
+    r = r < 0 ? r + Ny : (r >= Ny ? r - Ny : r);
+    return *reinterpret_cast<const P2*>(Gs + (size_t)r * ldg + u);
+  };
+  auto tpair = [&](int k, int u) -> P2 { return *reinterpret_cast<const P2*>(tp + (size_t)k * ldt + u); };
+  for (int cp = t; cp < N / 4; cp += NT) {
+    const int u = 2 * cp;
+    cpx<T> ph[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      ph[e] = mk<T>((T)(phx[2 * (u + e)] * sc), (T)(phx[2 * (u + e) + 1] * sc));
+    cpx<T> accv[R][2], accd[R][2];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) accv[i][e] = accd[i][e] = mk<T>(0, 0);
+    P2 L[NW], Rw[NW], tk[kNfK];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      L[j] = gpair(y0 - (kNfK - 1) + j, u);
+      Rw[j] = gpair(y0 + 1 + j, u);
+    }
+#pragma unroll
+    for (int kk = 0; kk < kNfK; ++kk) tk[kk] = tpair(kk, u);
+    for (int kb = 0; kb < hq; kb += kNfK) {
+      const bool more = kb + kNfK < hq;
+      P2 nL[kNfK], nR[kNfK], nt[kNfK];
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) {
+          nL[j] = gpair(y0 - kb - kNfK - (kNfK - 1) + j, u);
+          nR[j] = gpair(y0 + kb + kNfK + 1 + (R - 1) + j, u);
+          nt[j] = tpair(kb + kNfK + j, u);
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < kNfK; ++kk)
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const P2& l = L[i - kk + kNfK - 1];
+          const P2& r = Rw[i + kk];
+          const cpx<T> s0 = add(l.a, r.a), s1 = add(l.b, r.b);
+          accv[i][0] = mk<T>(accv[i][0].x + tk[kk].a.x * s0.x, accv[i][0].y + tk[kk].a.x * s0.y);
+          accd[i][0] = mk<T>(accd[i][0].x + tk[kk].a.y * s0.x, accd[i][0].y + tk[kk].a.y * s0.y);
+          accv[i][1] = mk<T>(accv[i][1].x + tk[kk].b.x * s1.x, accv[i][1].y + tk[kk].b.x * s1.y);
+          accd[i][1] = mk<T>(accd[i][1].x + tk[kk].b.y * s1.x, accd[i][1].y + tk[kk].b.y * s1.y);
+        }
+      if (more) {
+#pragma unroll
+        for (int j = NW - 1; j >= kNfK; --j) L[j] = L[j - kNfK];
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) L[j] = nL[j];
+#pragma unroll
+        for (int j = 0; j + kNfK < NW; ++j) Rw[j] = Rw[j + kNfK];
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) Rw[R - 1 + j] = nR[j];
+#pragma unroll
+        for (int j = 0; j < kNfK; ++j) tk[j] = nt[j];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      cpx<T>* buf = bufs + i * LDSN;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int ue = u + e;
+        const cpx<T> a = mul(accv[i][e], ph[e]), b = mul(accd[i][e], ph[e]);
+        if (ue == 0) {
+          buf[pad16(0)] = mk<T>(a.x, b.x);
+        } else {
+          buf[pad16(ue)] = mk<T>(a.x - b.y, a.y + b.x);
+          buf[pad16(N - ue)] = mk<T>(a.x + b.y, b.x - a.y);
+        }
+      }
+    }
+  }
+  if (t < R) bufs[t * LDSN + pad16(N / 2)] = mk<T>(0, 0);  // Nyquist column
+  __syncthreads();
+  const int g = t / G, tl = t % G;
+  fixed_fft<T, N, G, 16, true, 0, 1>(bufs + g * LDSN, QuarterTw<T>{qt, QL}, Pad16{}, tl);  // ends with a barrier
+  const T sh = shift ? (T)*shift : (T)0;
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  for (int sidx = t; sidx < (N / 2) * R; sidx += NT) {
+    const int tile = sidx / (2 * R), part = sidx % (2 * R);
+    const int row = part >> 1, x = 4 * tile + 2 * (part & 1), y = y0 + row;
+    const cpx<T>* b = bufs + row * LDSN;
+    const cpx<T> z0 = b[pad16(x)], z1 = b[pad16(x + 1)];
+    if (y < Ny) {
+      const v4 val = {z0.x + sh, z0.y, z1.x + sh, z1.y};
+      *reinterpret_cast<v4*>(out + tile_offset(x, y, N)) = val;
+    }
+  }
+}
+
 // The map's row spectrum with the compile-time FFT of M = Nx/2 points (one
 // row per workgroup): the row minus its own FP64 mean, packed as z[n] =
 // x[2n] + i x[2n+1], one forward DFT_M, then the split into the half
@@ -1312,6 +1434,20 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int l
       else if (lds_for(1) <= kLdsBytes) run(integral_constant<int, 1>{});
     }
   };
+  // widths that are not powers of two: G threads per row, R rows per
+  // workgroup (halved in FP64, whose rows take twice the LDS)
+  auto mix = [&](auto nc, auto gc, auto rc) {
+    constexpr int N = decltype(nc)::value, G = decltype(gc)::value;
+    constexpr int R = sizeof(T) == 4 ? decltype(rc)::value : (decltype(rc)::value > 1 ? decltype(rc)::value / 2 : 1);
+    const size_t lds = sizeof(cpx<T>) * ((size_t)N / 4 + (size_t)R * (N + N / 16));
+    if (lds > kLdsBytes) return;
+    auto k = k_node_fused_mix<T, N, R, G>;
+    allow_lds(k, lds);
+    const int nq = (Ny + R - 1) / R;
+    MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(G * R), lds, Gs, ldg, tp, ldt, hq,
+               cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, twiddles<T>(ctx, N));
+    done = true;
+  };
   using std::integral_constant;
   switch (Nx) {
     case 256: go(integral_constant<int, 256>{}); break;
@@ -1321,6 +1457,12 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int l
     case 4096: go(integral_constant<int, 4096>{}); break;
     case 8192: go(integral_constant<int, 8192>{}); break;
     case 16384: go(integral_constant<int, 16384>{}); break;
+    case 1000: mix(integral_constant<int, 1000>{}, integral_constant<int, 64>{}, integral_constant<int, 4>{}); break;
+    case 1280: mix(integral_constant<int, 1280>{}, integral_constant<int, 128>{}, integral_constant<int, 4>{}); break;
+    case 1536: mix(integral_constant<int, 1536>{}, integral_constant<int, 128>{}, integral_constant<int, 4>{}); break;
+    case 2000: mix(integral_constant<int, 2000>{}, integral_constant<int, 128>{}, integral_constant<int, 4>{}); break;
+    case 2560: mix(integral_constant<int, 2560>{}, integral_constant<int, 192>{}, integral_constant<int, 2>{}); break;
+    case 3072: mix(integral_constant<int, 3072>{}, integral_constant<int, 192>{}, integral_constant<int, 2>{}); break;
     default: break;
   }
   return done;
