@@ -246,7 +246,10 @@ __device__ __forceinline__ void c2r_fill(cpx<T>* buf, const cpx<T>* __restrict__
 // only the first (x0 + *shift) / second (x1) member of every node pair, from
 // in0 alone, for rows too long to hold two buffers in LDS.
 // ML > 0: the half length M = Nx/2 is the compile-time constant ML.
-template <class T, int G, int MODE, int ML>
+// GTW: the FFT's twiddles are read from the global table (through the
+// caches) instead of an LDS copy, so that the row buffers alone fill the LDS
+// and two workgroups fit on a CU (the 10000-wide node rows: 2 x 40 KB).
+template <class T, int G, int MODE, int ML, bool GTW = false>
 __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
                                                const cpx<T>* __restrict__ in1, T* __restrict__ out,
                                                int Nx, int Ny, const double* __restrict__ shift,
@@ -257,10 +260,18 @@ __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
   if (y >= Ny) return;
   if constexpr (ML > 0) Nx = 2 * ML;
   const int g = threadIdx.x, M = Nx >> 1, W = M + 1, QL = tw_len(M);
-  cpx<T>* q = reinterpret_cast<cpx<T>*>(smem);
-  cpx<T>* buf0 = q + QL;
+  const cpx<T>* q;
+  cpx<T>* buf0;
+  if constexpr (GTW) {
+    q = twM;
+    buf0 = reinterpret_cast<cpx<T>*>(smem);
+  } else {
+    cpx<T>* ql = reinterpret_cast<cpx<T>*>(smem);
+    for (int i = g; i < QL; i += G) ql[i] = twM[i];
+    q = ql;
+    buf0 = ql + QL;
+  }
   const NoPad P;
-  for (int i = g; i < QL; i += G) q[i] = twM[i];
   c2r_fill<T, G>(buf0, in0 + (size_t)y * W, M, twN, g);
   cpx<T>* buf1 = buf0 + M;
   if constexpr (MODE == 2) c2r_fill<T, G>(buf1, in1 + (size_t)y * W, M, twN, g);
@@ -1205,6 +1216,18 @@ void row_c2r(mlc_ctx* ctx, const cpx<T>* in0, const cpx<T>* in1, T* out, int Nx,
   }
   const size_t lds = row_lds<T>(M, MODE == 2 ? 2 : 1);
   if (lds > kLdsBytes) fail(MLC_ERR_UNSUPPORTED, "row FFT: row does not fit in LDS");
+  if constexpr (MODE == 2 && sizeof(T) == 4) {
+    if (M == 5000) {
+      // 10000-wide node rows: the twiddles from the global table, so that
+      // two workgroups (2 x 80 KB of row buffers) share a CU
+      const size_t lds2 = sizeof(cpx<T>) * 2 * (size_t)M;
+      auto k = k_row_c2r<T, 1024, 2, 5000, true>;
+      allow_lds(k, lds2);
+      MLC_LAUNCH(ctx, "fft_row_c2r_node", k, dim3(((Ny + 7) / 8) * 8), dim3(1024), lds2, in0, in1, out, Nx, Ny,
+                 shift, p, twM, twN);
+      return;
+    }
+  }
   with_row_group(M, "row FFT", [&](auto gc) {
     constexpr int G = decltype(gc)::value;
     with_fixed_len(M, [&](auto lc) {
