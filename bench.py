@@ -392,16 +392,23 @@ def time_batch_workload(ctx, which, steps):
     B = w["obs"].B
     samples = int((off[oi * B + B] - off[oi * B]).sum())
     ms, launches = kern["lc_batch"]
-    ab = samples * 4 * 8 + ncurves * (64 + 64)  # FP32 (mu, dmu/dr) taps; record read, accumulator written
+    # per step: FP32 (mu, dmu/dr) taps; record read, accumulator written (per
+    # launch of the per-band form: its share)
+    ab = samples * 4 * 8 + ncurves * (64 + 64)
     dom = max(kern, key=lambda k: kern[k][0])
+    max_pix = max(int(m.shape[0]) * int(m.shape[1]) for m in w["maps"])
+    lanes, gmul, flags = ctx.tune_lookup(ctx.TUNE_KIND_BATCH, "f32", w["obs"].nepoch / (w["obs"].nobs * B),
+                                         max_pix, with_flags=True)
     rec = {"ms_per_step": dt * 1e3, "light_curves_per_s": ncurves / dt, "curves": ncurves,
            "samples": samples, "maps": len(w["maps"]),
            "map_sizes": sorted({int(m.shape[0]) for m in w["maps"]}), "graph_replay_matches_eager": same,
            "steps": steps, "dominant_kernel": dom,
            "kernels_ms_per_step": {k: v[0] for k, v in sorted(kern.items())},
-           "lc_batch_roofline": {"bound": "hbm", "achieved": ab / (ms / launches / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
-                                 "unit": "GB/s", "frac": ab / (ms / launches / 1e3) / HBM_PEAK,
-                                 "algorithmic_bytes_per_launch": ab, "avg_launch_ms": ms / launches}}
+           "lc_batch_shape": {"lanes": lanes, "grid_multiple": gmul,
+                              "launches_per_step": launches, "one_launch_per_band": bool(flags & ctx.TUNE_BATCH_PER_BAND)},
+           "lc_batch_roofline": {"bound": "hbm", "achieved": ab / (ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
+                                 "unit": "GB/s", "frac": ab / (ms / 1e3) / HBM_PEAK,
+                                 "algorithmic_bytes_per_launch": ab / launches, "avg_launch_ms": ms / launches}}
     for sp in w["specs"]:
         sp.close()
     del w, eager, out, g

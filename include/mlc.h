@@ -359,11 +359,17 @@ int mlc_lc_chi2_grad_batch(mlc_ctx* ctx, int precision, int nmaps, const int* Nx
 #define MLC_TUNE_KIND_BATCH 1 /* the persistent batched launch */
 /* Force flag: the multi-node band kernel also for single-node bands. */
 #define MLC_TUNE_GENERAL_NODES 1
+/* Shape flag of the batched kernel: one launch per band (all maps) instead
+ * of every band in one persistent launch; a candidate of the tuner when a
+ * batched call has more than one band (it keeps one band's node maps in the
+ * caches at a time). */
+#define MLC_TUNE_BATCH_PER_BAND 2
 int mlc_tune_mode(mlc_ctx* ctx, int mode);
 int mlc_tune_force(mlc_ctx* ctx, int lanes, int grid_multiple, int flags);
-/* The kept shape for a key (lanes = grid_multiple = 0 when none is kept). */
+/* The kept shape for a key (lanes = grid_multiple = 0 when none is kept;
+ * *flags receives MLC_TUNE_BATCH_PER_BAND when that form was kept). */
 int mlc_tune_lookup(mlc_ctx* ctx, int kind, int precision, double epochs_per_band,
-                    double pixels, int* lanes, int* grid_multiple);
+                    double pixels, int* lanes, int* grid_multiple, int* flags);
 /* Kept shapes to / from a text file (loading adds to the kept shapes). */
 int mlc_tune_save(mlc_ctx* ctx, const char* path);
 int mlc_tune_load(mlc_ctx* ctx, const char* path);

@@ -72,7 +72,7 @@ SIGNATURES = {
                                     _i, _p, _i, _p, _p, _p, _p, _p]),
     "mlc_tune_mode": (_i, [_p, _i]),
     "mlc_tune_force": (_i, [_p, _i, _i, _i]),
-    "mlc_tune_lookup": (_i, [_p, _i, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mlc_tune_lookup": (_i, [_p, _i, _i, _d, _d, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mlc_tune_save": (_i, [_p, C.c_char_p]),
     "mlc_tune_load": (_i, [_p, C.c_char_p]),
     "mlc_lc_chi2_grad_noalloc": (_i, [_p, _i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p,

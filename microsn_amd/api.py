@@ -77,6 +77,7 @@ class Context:
     TUNE_OFF, TUNE_AUTO, TUNE_FORCE = 0, 1, 2
     TUNE_KIND_BANDS, TUNE_KIND_BATCH = 0, 1
     TUNE_GENERAL_NODES = 1
+    TUNE_BATCH_PER_BAND = 2
 
     def tune_mode(self, mode):
         self.check(lib().mlc_tune_mode(self.h, int(mode)))
@@ -86,12 +87,13 @@ class Context:
         self.check(lib().mlc_tune_force(self.h, int(lanes), int(grid_multiple), int(flags)))
         self.tune_mode(self.TUNE_FORCE)
 
-    def tune_lookup(self, kind, precision, epochs_per_band, pixels):
-        """(lanes, grid_multiple) kept for a key, or (0, 0)."""
-        la, gm = C.c_int(), C.c_int()
+    def tune_lookup(self, kind, precision, epochs_per_band, pixels, with_flags=False):
+        """(lanes, grid_multiple) kept for a key, or (0, 0); with_flags also
+        the shape flags (TUNE_BATCH_PER_BAND)."""
+        la, gm, fl = C.c_int(), C.c_int(), C.c_int()
         self.check(lib().mlc_tune_lookup(self.h, int(kind), _prec(precision), float(epochs_per_band),
-                                         float(pixels), C.byref(la), C.byref(gm)))
-        return la.value, gm.value
+                                         float(pixels), C.byref(la), C.byref(gm), C.byref(fl)))
+        return (la.value, gm.value, fl.value) if with_flags else (la.value, gm.value)
 
     def tune_save(self, path):
         self.check(lib().mlc_tune_save(self.h, str(path).encode()))

@@ -60,7 +60,10 @@ struct mlc_ctx {
   // the forced shape, and the kept shape (lanes, grid multiple) per key
   int tune_mode = MLC_TUNE_AUTO;
   int tune_force_lanes = 16, tune_force_gmul = 1, tune_force_flags = 0;
-  std::map<uint64_t, std::pair<int, int>> tune_cache;
+  struct TunedShape {
+    int lanes, gmul, per_band;
+  };
+  std::map<uint64_t, TunedShape> tune_cache;
 
   // per-kernel timing
   bool timing = false;

@@ -653,8 +653,10 @@ __global__ __launch_bounds__(256) void k_lc_band(
       const int vi = li / (LANES / 8);
       const double sc = vi == 7 ? -2.0 : (vi == 5 || vi == 6) ? bsc : 1.0;
       double* a = acc + (size_t)(pos0 + i) * 8 + vi;
+      // an explicit fma: the batched kernel forms the same sum in registers
+      // or in memory, and contraction must not differ between them
       if (FIRST) *a = sc * val;
-      else *a += sc * val;
+      else *a = fma(sc, val, *a);
       if (vi == 0) {
         if (FIRST) bad[pos0 + i] = gbad;
         else bad[pos0 + i] |= gbad;
@@ -727,12 +729,14 @@ __global__ __launch_bounds__(1024) void k_batch_groups(const int* __restrict__ c
 // curve give the same per-band group sums; band 0 sets the FP64
 // accumulator, later bands add), keeping the accumulator in a register and
 // writing it once.  qctr: 8 counters, zero at launch.
-template <class T, int LANES>
+// PER_BAND: only band bsel, added to (bsel == 0: written into) the
+// accumulator in memory, as one launch per band does it.
+template <class T, int LANES, bool PER_BAND>
 __global__ __launch_bounds__(256) void k_lc_batch(
     const BatchJob* __restrict__ jobs, int B, const CurveRec* __restrict__ rec, const int* __restrict__ start,
     const int* __restrict__ cnt, const int* __restrict__ gstart, int nmp, int nobs,
     const int* __restrict__ obs_off, int nepoch, const Epoch* __restrict__ ep, const double2* __restrict__ rng,
-    int* __restrict__ qctr, double* __restrict__ acc, int* __restrict__ bad) {
+    int* __restrict__ qctr, double* __restrict__ acc, int* __restrict__ bad, int bsel) {
   constexpr int GPW = 64 / LANES;
   constexpr bool kF32 = sizeof(T) == 4;
   __shared__ LnEntry lnt[kF32 ? kLnTab : 1];
@@ -772,8 +776,10 @@ __global__ __launch_bounds__(256) void k_lc_batch(
         CurveRec R{};
         if (act) R = rec[i];
         double a8 = 0.0;  // accumulator entry vi of this lane's curve
+        double psc = 0.0, pval = 0.0;  // PER_BAND: the band's term sc * val
         bool gbad = false;
-        for (int b = 0; b < B; ++b) {
+        const int b0 = PER_BAND ? bsel : 0, b1 = PER_BAND ? bsel + 1 : B;
+        for (int b = b0; b < b1; ++b) {
           const BatchJob* J = jobs + (size_t)mp * B + b;
           const int Nx = J->Nx, Ny = J->Ny, cntn = J->cnt;
           const double bsc = J->bsc;
@@ -799,12 +805,19 @@ __global__ __launch_bounds__(256) void k_lc_batch(
           const int vi = li / (LANES / 8);
           const double sc = vi == 7 ? -2.0 : (vi == 5 || vi == 6) ? bsc : 1.0;
           if (b == 0) a8 = sc * val;
-          else a8 += sc * val;
+          else a8 = fma(sc, val, a8);
+          psc = sc;
+          pval = val;
         }
         if (act && li % (LANES / 8) == 0) {
           const int vi = li / (LANES / 8);
-          acc[(size_t)i * 8 + vi] = a8;
-          if (vi == 0) bad[i] = gbad;
+          if (!PER_BAND || bsel == 0) {
+            acc[(size_t)i * 8 + vi] = a8;
+            if (vi == 0) bad[i] = gbad;
+          } else {
+            acc[(size_t)i * 8 + vi] = fma(psc, pval, acc[(size_t)i * 8 + vi]);  // as k_lc_band adds it
+            if (vi == 0) bad[i] |= gbad;
+          }
         }
       }
     }
@@ -1005,6 +1018,7 @@ void validate_fused(const Args& a) {
 // same bits every time.
 struct LaunchShape {
   int lanes = kDefaultLanes, gmul = 1;
+  int per_band = 0;  // batched kernel: one launch per band
 };
 enum TuneKind { kTuneBands = MLC_TUNE_KIND_BANDS, kTuneBatch = MLC_TUNE_KIND_BATCH };
 
@@ -1043,18 +1057,22 @@ LaunchShape tuned_shape(mlc_ctx* ctx, uint64_t key) {
   if (ctx->tune_mode == MLC_TUNE_FORCE) {
     sh.lanes = ctx->tune_force_lanes;
     sh.gmul = ctx->tune_force_gmul;
+    sh.per_band = (ctx->tune_force_flags & MLC_TUNE_BATCH_PER_BAND) != 0;
   } else if (ctx->tune_mode == MLC_TUNE_AUTO) {
     auto it = ctx->tune_cache.find(key);
     if (it != ctx->tune_cache.end()) {
-      sh.lanes = it->second.first;
-      sh.gmul = it->second.second;
+      sh.lanes = it->second.lanes;
+      sh.gmul = it->second.gmul;
+      sh.per_band = it->second.per_band;
     }
   }
   return sh;
 }
 
-// Times run(shape) for every candidate and keeps the fastest under `key`.
-LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(LaunchShape)>& run) {
+// Times run(shape) for every candidate and keeps the fastest under `key`
+// (with per_band_too, each lane and grid choice also as one launch per band).
+LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(LaunchShape)>& run,
+                      bool per_band_too = false) {
   struct Restore {  // per-kernel timing is off while the candidates run
     mlc_ctx* c;
     bool t;
@@ -1071,10 +1089,12 @@ LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(Launc
   LaunchShape best;
   float best_ms = INFINITY;
   for (int lanes : {8, 16, 32})
-    for (int gmul : {1, 2}) {
+    for (int gmul : {1, 2})
+      for (int pb = 0; pb <= (per_band_too ? 1 : 0); ++pb) {
       LaunchShape c;
       c.lanes = lanes;
       c.gmul = gmul;
+      c.per_band = pb;
       run(c);  // first run: caches and clocks settle
       hip_check(hipEventRecord(rs.e[0], ctx->stream), "hipEventRecord");
       run(c);
@@ -1087,7 +1107,7 @@ LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(Launc
         best = c;
       }
     }
-  ctx->tune_cache[key] = {best.lanes, best.gmul};
+  ctx->tune_cache[key] = {best.lanes, best.gmul, best.per_band};
   return best;
 }
 
@@ -1326,27 +1346,39 @@ void run_batch(mlc_ctx* ctx, const BatchArgs& a) {
   if (a.nepoch > 0)
     MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
                a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
+  // every band in one persistent launch, or (sh.per_band) one launch per
+  // band over all maps; each launch starts from a fresh work queue
   auto launch = [&](LaunchShape sh) {
     with_lanes(sh.lanes, [&](auto lc) {
       constexpr int L = decltype(lc)::value;
-      MLC_LAUNCH(ctx, "lc_batch_groups", k_batch_groups, dim3(1), dim3(1024), 0, cnt, NP, 64 / L, gstart, qctr);
-      auto go = [&](auto tag) {
+      auto go = [&](auto tag, auto pbc) {
         using TT = decltype(tag);
-        auto k = k_lc_batch<TT, L>;
+        constexpr bool PB = decltype(pbc)::value;
+        auto k = k_lc_batch<TT, L, PB>;
         const int64_t want = cdiv(n, 4 * (64 / L));
         const int grid = (int)(cdiv(std::min<int64_t>(want, (int64_t)sh.gmul *
                                                                 fill_grid(ctx, resident_blocks((const void*)k, 256))),
                                     8) *
                                8);
-        MLC_LAUNCH(ctx, "lc_batch", k, dim3(grid), dim3(256), 0, d_jobs, B, rec, start, cnt, gstart, NP, a.nobs,
-                   a.obs_off, a.nepoch, ep, rng, qctr, acc, bad);
+        for (int b = 0; b < (PB ? B : 1); ++b) {
+          MLC_LAUNCH(ctx, "lc_batch_groups", k_batch_groups, dim3(1), dim3(1024), 0, cnt, NP, 64 / L, gstart,
+                     qctr);
+          MLC_LAUNCH(ctx, "lc_batch", k, dim3(grid), dim3(256), 0, d_jobs, B, rec, start, cnt, gstart, NP, a.nobs,
+                     a.obs_off, a.nepoch, ep, rng, qctr, acc, bad, b);
+        }
       };
-      if (a.precision == MLC_F64) go(double{});
-      else go(float{});
+      using std::integral_constant;
+      if (a.precision == MLC_F64) {
+        if (sh.per_band) go(double{}, integral_constant<bool, true>{});
+        else go(double{}, integral_constant<bool, false>{});
+      } else {
+        if (sh.per_band) go(float{}, integral_constant<bool, true>{});
+        else go(float{}, integral_constant<bool, false>{});
+      }
     });
   };
   const uint64_t key = tune_key(kTuneBatch, a.precision, (double)a.nepoch / ((double)a.nobs * B), (double)max_pix);
-  if (needs_tuning(ctx, key)) tune_pick(ctx, key, launch);
+  if (needs_tuning(ctx, key)) tune_pick(ctx, key, launch, B > 1);
   launch(tuned_shape(ctx, key));
   MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, NP, start, pos_of, a.params,
              acc, bad, a.chi2, a.grad);
@@ -1525,7 +1557,7 @@ int mlc_tune_force(mlc_ctx* ctx, int lanes, int gmul, int flags) {
   return guarded(ctx, [&] {
     MLC_REQUIRE(lanes == 8 || lanes == 16 || lanes == 32, "mlc_tune_force: lanes must be 8, 16 or 32");
     MLC_REQUIRE(gmul >= 1 && gmul <= 8, "mlc_tune_force: grid multiple must be 1..8");
-    MLC_REQUIRE((flags & ~MLC_TUNE_GENERAL_NODES) == 0, "mlc_tune_force: unknown flag");
+    MLC_REQUIRE((flags & ~(MLC_TUNE_GENERAL_NODES | MLC_TUNE_BATCH_PER_BAND)) == 0, "mlc_tune_force: unknown flag");
     ctx->tune_force_lanes = lanes;
     ctx->tune_force_gmul = gmul;
     ctx->tune_force_flags = flags;
@@ -1533,13 +1565,15 @@ int mlc_tune_force(mlc_ctx* ctx, int lanes, int gmul, int flags) {
 }
 
 int mlc_tune_lookup(mlc_ctx* ctx, int kind, int precision, double epochs_per_band, double pixels, int* lanes,
-                    int* gmul) {
+                    int* gmul, int* flags) {
   return guarded(ctx, [&] {
-    MLC_REQUIRE(lanes && gmul, "mlc_tune_lookup: null output");
+    MLC_REQUIRE(lanes && gmul && flags, "mlc_tune_lookup: null output");
     MLC_REQUIRE(kind == MLC_TUNE_KIND_BANDS || kind == MLC_TUNE_KIND_BATCH, "mlc_tune_lookup: bad kind");
     auto it = ctx->tune_cache.find(tune_key(kind, precision, epochs_per_band, pixels));
-    *lanes = it == ctx->tune_cache.end() ? 0 : it->second.first;
-    *gmul = it == ctx->tune_cache.end() ? 0 : it->second.second;
+    const bool hit = it != ctx->tune_cache.end();
+    *lanes = hit ? it->second.lanes : 0;
+    *gmul = hit ? it->second.gmul : 0;
+    *flags = hit && it->second.per_band ? MLC_TUNE_BATCH_PER_BAND : 0;
   });
 }
 
@@ -1548,9 +1582,10 @@ int mlc_tune_save(mlc_ctx* ctx, const char* path) {
     MLC_REQUIRE(path, "mlc_tune_save: null path");
     FILE* f = std::fopen(path, "w");
     MLC_REQUIRE(f, std::string("mlc_tune_save: cannot write ") + path);
-    std::fprintf(f, "# mlc launch shapes: key lanes grid_multiple\n");
+    std::fprintf(f, "# mlc launch shapes: key lanes grid_multiple per_band\n");
     for (auto& kv : ctx->tune_cache)
-      std::fprintf(f, "%llu %d %d\n", (unsigned long long)kv.first, kv.second.first, kv.second.second);
+      std::fprintf(f, "%llu %d %d %d\n", (unsigned long long)kv.first, kv.second.lanes, kv.second.gmul,
+                   kv.second.per_band);
     const bool bad = std::fclose(f) != 0;
     MLC_REQUIRE(!bad, std::string("mlc_tune_save: write failed: ") + path);
   });
@@ -1561,18 +1596,19 @@ int mlc_tune_load(mlc_ctx* ctx, const char* path) {
     MLC_REQUIRE(path, "mlc_tune_load: null path");
     FILE* f = std::fopen(path, "r");
     MLC_REQUIRE(f, std::string("mlc_tune_load: cannot read ") + path);
-    std::map<uint64_t, std::pair<int, int>> got;
+    std::map<uint64_t, mlc_ctx::TunedShape> got;
     char line[256];
     bool ok = true;
     while (std::fgets(line, sizeof line, f)) {
       if (line[0] == '#' || line[0] == '\n') continue;
       unsigned long long k = 0;
-      int l = 0, g = 0;
-      if (std::sscanf(line, "%llu %d %d", &k, &l, &g) != 3 || !(l == 8 || l == 16 || l == 32) || g < 1 || g > 8) {
+      int l = 0, g = 0, pb = 0;
+      const int nf = std::sscanf(line, "%llu %d %d %d", &k, &l, &g, &pb);
+      if (nf < 3 || !(l == 8 || l == 16 || l == 32) || g < 1 || g > 8 || pb < 0 || pb > 1) {
         ok = false;
         break;
       }
-      got[(uint64_t)k] = {l, g};
+      got[(uint64_t)k] = {l, g, pb};
     }
     std::fclose(f);
     MLC_REQUIRE(ok, std::string("mlc_tune_load: malformed line in ") + path);

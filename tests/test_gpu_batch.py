@@ -1,5 +1,6 @@
 """The batched light-curve call (mlc_lc_chi2_grad_batch: several maps of
-different sizes, all bands in one persistent launch) and the launch-shape
+different sizes, all bands in one persistent launch or one launch per band)
+and the launch-shape
 tuner, through the C ABI:
   * one map: the batched call equals the per-band launches of
     mlc_lc_chi2_grad bit for bit at every launch shape (8, 16, 32 lanes per
@@ -8,8 +9,11 @@ tuner, through the C ABI:
     and multi-node profiles, curves with invalid map or profile indices:
     every curve equals the per-map call on its own map bit for bit;
   * a captured batched call replays bit-equal to the eager call;
+  * every batched case also as one launch per band (the tuner's other
+    launch form), bit-equal to the persistent launch;
   * the tuner keeps one of its candidates per key, the result equals a
-    forced run of that shape, and kept shapes survive a save/load;
+    forced run of that shape, and kept shapes (with the batched launch
+    form) survive a save/load;
   * the multi-node band kernel forced onto single-node bands (radius below
     the node, and a NaN radius) agrees with the single-node kernel."""
 import numpy as np
@@ -69,14 +73,15 @@ def _t(x):
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("lanes", [8, 16, 32])
-def test_batch_one_map_equals_band_launches(tuned, prec, lanes):
+@pytest.mark.parametrize("per_band", [False, True])
+def test_batch_one_map_equals_band_launches(tuned, prec, lanes, per_band):
     ctx = tuned
     rng = np.random.default_rng(100 + lanes)
     spec, nodes = _nodes(ctx, _map(ctx, 512, 5), prec)
     obs = _obs(ctx, rng, 9, 2)
     n = 20000
     prm, prof, oi = _curves(rng, n, [512], np.zeros(n, int), 9, 2)
-    ctx.tune_force(lanes, 1)
+    ctx.tune_force(lanes, 1, ctx.TUNE_BATCH_PER_BAND if per_band else 0)
     P, pr, o = _t(prm), _t(prof), _t(oi)
     ref = lcm.chi2_grad(ctx, nodes, obs, P, pr, o)
     got = lcm.chi2_grad_batch(ctx, lcm.NodeBatch([nodes]), obs, P, torch.zeros(n, dtype=torch.int32,
@@ -88,7 +93,8 @@ def test_batch_one_map_equals_band_launches(tuned, prec, lanes):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_batch_many_maps_equals_per_map_calls(tuned, prec):
+@pytest.mark.parametrize("per_band", [False, True])
+def test_batch_many_maps_equals_per_map_calls(tuned, prec, per_band):
     ctx = tuned
     rng = np.random.default_rng(7)
     sizes = [256, 384, 512, 320, 640]
@@ -103,7 +109,7 @@ def test_batch_many_maps_equals_per_map_calls(tuned, prec):
     midx[::97] = -1
     midx[5::97] = len(sizes)
     prm, prof, oi = _curves(rng, n, sizes, midx, 13, 2)
-    ctx.tune_force(16, 1)
+    ctx.tune_force(16, 1, ctx.TUNE_BATCH_PER_BAND if per_band else 0)
     c, g = lcm.chi2_grad_batch(ctx, lcm.NodeBatch(sets), obs, _t(prm), _t(midx), _t(prof), _t(oi))
     torch.cuda.synchronize()
     bad = (midx < 0) | (midx >= len(sizes))
@@ -118,8 +124,11 @@ def test_batch_many_maps_equals_per_map_calls(tuned, prec):
         s.close()
 
 
-def test_batch_graph_replay_equals_eager(tuned):
+@pytest.mark.parametrize("mode", ["auto", "per_band"])
+def test_batch_graph_replay_equals_eager(tuned, mode):
     ctx = tuned
+    if mode == "per_band":
+        ctx.tune_force(16, 1, ctx.TUNE_BATCH_PER_BAND)
     rng = np.random.default_rng(8)
     sizes = [256, 512]
     specs, sets = zip(*[_nodes(ctx, _map(ctx, N, 70 + i), "f32") for i, N in enumerate(sizes)])
@@ -191,6 +200,48 @@ def test_tuner_keeps_a_candidate_and_reuses_it(tuned, tmp_path):
     finally:
         other.close()
     spec.close()
+
+
+def test_tuner_batch_kind_keeps_the_launch_form(tuned, tmp_path):
+    """A two-band batched call under the tuner keeps lanes, grid multiple and
+    the launch form (persistent or one launch per band) for its key; the
+    result equals a forced run of the kept shape and the form survives a
+    save/load."""
+    ctx = tuned
+    rng = np.random.default_rng(12)
+    sizes = [192, 160]
+    specs, sets = zip(*[_nodes(ctx, _map(ctx, N, 90 + i), "f64") for i, N in enumerate(sizes)])
+    batch = lcm.NodeBatch(sets)
+    # 7 epochs per band, largest map 192^2: a key no other test uses
+    sets_obs = [[(np.sort(rng.uniform(0, 10, 7)), rng.normal(-1.7, .2, 7), np.full(7, .05)) for _ in range(2)]]
+    obs = lcm.Observations(ctx, 2, sets_obs)
+    n = 5000
+    midx = rng.integers(0, 2, n).astype(np.int32)
+    prm, prof, oi = _curves(rng, n, sizes, midx, 1, 2)
+    args = (_t(prm), _t(midx), _t(prof), _t(oi))
+    ctx.tune_mode(ctx.TUNE_AUTO)
+    key = (ctx.TUNE_KIND_BATCH, "f64", 7, 192 * 192)
+    assert ctx.tune_lookup(*key, with_flags=True) == (0, 0, 0)
+    auto = lcm.chi2_grad_batch(ctx, batch, obs, *args)
+    lanes, gm, fl = ctx.tune_lookup(*key, with_flags=True)
+    assert lanes in (8, 16, 32) and gm in (1, 2) and fl in (0, ctx.TUNE_BATCH_PER_BAND)
+    ctx.tune_force(lanes, gm, fl)
+    forced = lcm.chi2_grad_batch(ctx, batch, obs, *args)
+    ctx.tune_force(lanes, gm, fl ^ ctx.TUNE_BATCH_PER_BAND)
+    other_form = lcm.chi2_grad_batch(ctx, batch, obs, *args)
+    torch.cuda.synchronize()
+    for x, y in ((auto, forced), (auto, other_form)):
+        assert torch.equal(x[0], y[0]) and torch.equal(x[1], y[1])
+    path = tmp_path / "shapes.txt"
+    ctx.tune_save(path)
+    other = api.Context(0)
+    try:
+        other.tune_load(path)
+        assert other.tune_lookup(*key, with_flags=True) == (lanes, gm, fl)
+    finally:
+        other.close()
+    for sp in specs:
+        sp.close()
 
 
 def test_multi_node_kernel_on_single_node_bands(tuned):
