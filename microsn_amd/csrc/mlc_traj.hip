@@ -387,60 +387,75 @@ struct NodeView {
   NodeAt node;
 };
 
-// One epoch (time te, observed magnitude em, 1/sigma eis) of curve R on the
-// node maps of one (profile, band): adds its eight FP64 terms to s and
-// raises flag when mu is not positive and finite.  The caller has checked
-// that the position lies in the pixel-centre hull (0 <= x <= Nx - 1,
+// One epoch of a curve between its tap loads and its sums: the epoch
+// (time te, observed magnitude em, 1/sigma eis), the bilinear weights and
+// the raw corner data of the node maps as loaded.  Splitting a sample there
+// lets a lane issue the loads of its next epoch before it finishes this one.
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+template <class T>
+struct SampleTaps {
+  double te, em, eis;
+  T fx, fy;
+  T dr;           // single node: r - node radius
+  T w[8];         // several nodes: mu and dmu/dr weights of (Ma, Da, Mb, Db), see lc_fetch
+  f4a8 top, bot;  // FP32 single node: 16-byte loads of the cell's rows
+  bool cross;     // FP32 single node: the cell crosses a tile column
+  cpx<T> c[8];    // FP32 single node: c[1], c[3] the right taps of a crossing cell;
+                  // FP64 single node: the four corners; several nodes: Ma[0..3], Mb[0..3]
+};
+
+// The first half of a sample of curve R at epoch (te, em, eis) on the node
+// maps of one (profile, band): position, radius, bilinear weights, the
+// node interval and its Hermite weights, and the tap loads.  The caller has
+// checked that the position lies in the pixel-centre hull (0 <= x <= Nx - 1,
 // 0 <= y <= Ny - 1).  ONE: a single radius node, first-order Taylor step in
 // r, taps through 32-bit offsets from node(0) (the map is below 4 GiB).
 // rowstep = 4 Nx - 12, the pair-offset step from pixel row 4k+3 to 4k+4.
+// lc_fetch and lc_accum fuse only where they call fma: their rounding does
+// not depend on how the compiler schedules the code around them.
 template <class T, bool ONE, class NodeAt>
-__device__ __forceinline__ void lc_sample(const NodeView<NodeAt>& nv, int Nx, int Ny, int rowstep, double bsc,
-                                          const CurveRec& R, double te, double em, double eis,
-                                          const LnEntry* lnt, double (&s)[8], bool& flag) {
+__device__ __forceinline__ SampleTaps<T> lc_fetch(const NodeView<NodeAt>& nv, int Nx, int Ny, int rowstep,
+                                                  double bsc, const CurveRec& R, double te, double em,
+                                                  double eis) {
+#pragma clang fp contract(off)
   constexpr bool kF32 = sizeof(T) == 4;
-  constexpr double kMag = -1.08573620475812959810;  // -2.5 / ln 10
-  constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
+  SampleTaps<T> P;
+  P.te = te;
+  P.em = em;
+  P.eis = eis;
   const double x = fma(R.vct, te, R.x0), y = fma(R.vst, te, R.y0);
-  const double r = bsc * fma(R.w, te, R.r0);
+  const double rt = fma(R.w, te, R.r0), r = bsc * rt;
   const int ix = min((int)x, Nx - 2), iy = min((int)y, Ny - 2);
-  const T fx = (T)(x - ix), fy = (T)(y - iy), gx = (T)1 - fx, gy = (T)1 - fy;
+  P.fx = (T)(x - ix);
+  P.fy = (T)(y - iy);
   const int o00 = tile32(ix, iy, Nx);
   const int sx = (ix & 3) == 3 ? 13 : 1, sy = (iy & 3) == 3 ? rowstep : 4;
   const int o01 = o00 + sx, o10 = o00 + sy, o11 = o10 + sx;
-  // corner values u (mu) and v (dmu/dr) from one or two radius nodes
-  T u[4], v[4];
   if constexpr (ONE) {  // first-order Taylor step about the only node
-    const T dr = (T)(r - nv.rlo);
+    P.dr = (T)fma(bsc, rt, -nv.rlo);
     const char* bb = reinterpret_cast<const char*>(nv.node(0));
     auto at = [&](int o) -> const cpx<T>* {
       return reinterpret_cast<const cpx<T>*>(bb + (uint32_t)o * (uint32_t)sizeof(cpx<T>));
     };
-    cpx<T> c0, c1, c2, c3;
     if constexpr (kF32) {
       // the left and right taps of a row are neighbours in the tile unless
       // ix % 4 == 3: one 16-byte load per row, and a second tap load only on
       // lanes whose cell crosses a tile column (the 16-byte load stays
-      // inside the map: ix + 1 <= Nx - 1)
-      typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
-      const f4a8 top = *reinterpret_cast<const f4a8*>(at(o00));
-      const f4a8 bot = *reinterpret_cast<const f4a8*>(at(o10));
-      c0 = cpx<T>{top.x, top.y};
-      c2 = cpx<T>{bot.x, bot.y};
-      if (sx == 1) {
-        c1 = cpx<T>{top.z, top.w};
-        c3 = cpx<T>{bot.z, bot.w};
-      } else {
-        c1 = *at(o01);
-        c3 = *at(o11);
+      // inside the map: ix + 1 <= Nx - 1).  The crossing taps go to
+      // registers of their own, so their loads need not wait for the
+      // 16-byte loads to land.
+      P.top = *reinterpret_cast<const f4a8*>(at(o00));
+      P.bot = *reinterpret_cast<const f4a8*>(at(o10));
+      P.cross = sx != 1;
+      P.c[1] = cpx<T>{};
+      P.c[3] = cpx<T>{};
+      if (P.cross) {
+        P.c[1] = *at(o01);
+        P.c[3] = *at(o11);
       }
     } else {
-      c0 = *at(o00); c1 = *at(o01); c2 = *at(o10); c3 = *at(o11);
+      P.c[0] = *at(o00); P.c[1] = *at(o01); P.c[2] = *at(o10); P.c[3] = *at(o11);
     }
-    u[0] = c0.x + dr * c0.y; v[0] = c0.y;
-    u[1] = c1.x + dr * c1.y; v[1] = c1.y;
-    u[2] = c2.x + dr * c2.y; v[2] = c2.y;
-    u[3] = c3.x + dr * c3.y; v[3] = c3.y;
   } else {
     // node interval k, k+1; Taylor about the end node outside [rlo, rhi].
     // A NaN radius counts as below the first node (its Taylor step gives a
@@ -452,35 +467,74 @@ __device__ __forceinline__ void lc_sample(const NodeView<NodeAt>& nv, int Nx, in
     const bool lo = !(r > nv.rlo), hi = !lo && r >= nv.rhi;
     const int ka = hi ? nn - 1 : k, kb = (lo || hi) ? ka : k + 1;
     const double ra = nv.rad[ka], h = nv.rad[kb] - ra;
-    T wa, wda, wb, wdb, za, zda, zb, zdb;  // mu and dmu/dr weights of (Ma, Da, Mb, Db)
+    T* w = P.w;  // wa, wda, wb, wdb, za, zda, zb, zdb
     if (ka == kb) {
       const T dr = (T)(r - ra);
-      wa = 1; wda = dr; wb = 0; wdb = 0;
-      za = 0; zda = 1; zb = 0; zdb = 0;
+      w[0] = 1; w[1] = dr; w[2] = 0; w[3] = 0;
+      w[4] = 0; w[5] = 1; w[6] = 0; w[7] = 0;
     } else {
       // FP32 maps: 1/h from the table; FP64 maps keep the division
       const double ih = kF32 ? nv.ih[ka] : 1.0 / h;
       const T S = (T)(kF32 ? (r - ra) * ih : (r - ra) / h), S2 = S * S, S3 = S2 * S;
       const T H = (T)h, iH = (T)ih;
-      wa = 2 * S3 - 3 * S2 + 1; wda = (S3 - 2 * S2 + S) * H;
-      wb = -2 * S3 + 3 * S2;    wdb = (S3 - S2) * H;
-      za = (6 * S2 - 6 * S) * iH; zda = 3 * S2 - 4 * S + 1;
-      zb = (-6 * S2 + 6 * S) * iH; zdb = 3 * S2 - 2 * S;
+      w[0] = 2 * S3 - 3 * S2 + 1; w[1] = (S3 - 2 * S2 + S) * H;
+      w[2] = -2 * S3 + 3 * S2;    w[3] = (S3 - S2) * H;
+      w[4] = (6 * S2 - 6 * S) * iH; w[5] = 3 * S2 - 4 * S + 1;
+      w[6] = (-6 * S2 + 6 * S) * iH; w[7] = 3 * S2 - 2 * S;
     }
     const cpx<T>* ma = nv.node(ka);
     const cpx<T>* mb = nv.node(kb);
     const int off[4] = {o00, o01, o10, o11};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const cpx<T> A = ma[off[q]], Bq = mb[off[q]];
-      u[q] = wa * A.x + wda * A.y + wb * Bq.x + wdb * Bq.y;
-      v[q] = za * A.x + zda * A.y + zb * Bq.x + zdb * Bq.y;
+      P.c[q] = ma[off[q]];
+      P.c[4 + q] = mb[off[q]];
     }
   }
-  const T mu_t = gy * (gx * u[0] + fx * u[1]) + fy * (gx * u[2] + fx * u[3]);
-  const T mx_t = gy * (u[1] - u[0]) + fy * (u[3] - u[2]);
-  const T my_t = gx * (u[2] - u[0]) + fx * (u[3] - u[1]);
-  const T mr_t = gy * (gx * v[0] + fx * v[1]) + fy * (gx * v[2] + fx * v[3]);
+  return P;
+}
+
+// The second half of a sample: adds its eight FP64 terms to s and raises
+// flag when mu is not positive and finite.
+template <class T, bool ONE>
+__device__ __forceinline__ void lc_accum(const SampleTaps<T>& P, const CurveRec& R, const LnEntry* lnt,
+                                         double (&s)[8], bool& flag) {
+#pragma clang fp contract(off)
+  constexpr bool kF32 = sizeof(T) == 4;
+  constexpr double kMag = -1.08573620475812959810;  // -2.5 / ln 10
+  constexpr double kDc = 2.17147240951625919620;    //  5 / ln 10
+  const T fx = P.fx, fy = P.fy, gx = (T)1 - fx, gy = (T)1 - fy;
+  // corner values u (mu) and v (dmu/dr) from one or two radius nodes
+  T u[4], v[4];
+  if constexpr (ONE) {
+    const T dr = P.dr;
+    cpx<T> c0, c1, c2, c3;
+    if constexpr (kF32) {
+      c0 = cpx<T>{P.top.x, P.top.y};
+      c2 = cpx<T>{P.bot.x, P.bot.y};
+      c1 = P.cross ? P.c[1] : cpx<T>{P.top.z, P.top.w};
+      c3 = P.cross ? P.c[3] : cpx<T>{P.bot.z, P.bot.w};
+    } else {
+      c0 = P.c[0]; c1 = P.c[1]; c2 = P.c[2]; c3 = P.c[3];
+    }
+    u[0] = fma(dr, c0.y, c0.x); v[0] = c0.y;
+    u[1] = fma(dr, c1.y, c1.x); v[1] = c1.y;
+    u[2] = fma(dr, c2.y, c2.x); v[2] = c2.y;
+    u[3] = fma(dr, c3.y, c3.x); v[3] = c3.y;
+  } else {
+    const T* w = P.w;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const cpx<T> A = P.c[q], Bq = P.c[4 + q];
+      u[q] = fma(w[3], Bq.y, fma(w[2], Bq.x, fma(w[1], A.y, w[0] * A.x)));
+      v[q] = fma(w[7], Bq.y, fma(w[6], Bq.x, fma(w[5], A.y, w[4] * A.x)));
+    }
+  }
+  // bilinear value and slopes: gy (gx u0 + fx u1) + fy (gx u2 + fx u3), ...
+  const T mu_t = fma(gy, fma(gx, u[0], fx * u[1]), fy * fma(gx, u[2], fx * u[3]));
+  const T mx_t = fma(gy, u[1] - u[0], fy * (u[3] - u[2]));
+  const T my_t = fma(gx, u[2] - u[0], fx * (u[3] - u[1]));
+  const T mr_t = fma(gy, fma(gx, v[0], fx * v[1]), fy * fma(gx, v[2], fx * v[3]));
   flag |= !(mu_t > (T)0 && mu_t < (T)INFINITY);
   double lnmu, rmu;
   if constexpr (kF32) {
@@ -490,7 +544,8 @@ __device__ __forceinline__ void lc_sample(const NodeView<NodeAt>& nv, int Nx, in
     lnmu = ln_pos(mu_t);
     rmu = 1.0 / mu_t;
   }
-  const double rho = (em - fma(kMag, lnmu, R.dm)) * eis;
+  const double te = P.te, eis = P.eis;
+  const double rho = (P.em - fma(kMag, lnmu, R.dm)) * eis;
   const double q = rho * eis;
   const double dc = kDc * q * rmu, tdc = dc * te;
   const double mx = (double)mx_t, my = (double)my_t, mr = (double)mr_t;
@@ -511,7 +566,12 @@ __device__ __forceinline__ void lc_sample(const NodeView<NodeAt>& nv, int Nx, in
 // groups agree on a set whose band-b segment has at most kStageEpochs
 // epochs), otherwise from global memory one record ahead.  st is uniform
 // per wavefront.
-template <class T, bool ONE, int LANES, class NodeAt>
+// PIPE: two epochs in flight per lane (the taps of the next are loaded while
+// this one's sums are formed; P and Q alternate, so no copies).  It pays
+// with several nodes (eight tap loads a sample) and on FP64 maps, 6-8% a
+// launch; the FP32 single-node kernel is as fast without it at 16 lanes and
+// slower at 32 (128 VGPRs).  The results are the same either way.
+template <class T, bool ONE, int LANES, bool PIPE = !(ONE && sizeof(T) == 4), class NodeAt>
 __device__ __forceinline__ void lc_band_epochs(const NodeView<NodeAt>& nv, int Nx, int Ny, double bsc,
                                                const CurveRec& R, bool act, bool ok, int e0, int e1, int b,
                                                int B, int nobs, const int* __restrict__ obs_off, int nepoch,
@@ -531,16 +591,66 @@ __device__ __forceinline__ void lc_band_epochs(const NodeView<NodeAt>& nv, int N
   const int jend = ok ? e1 : e0;
   int j = e0 + li;
   if (__all(!ok || R.o * B + b == st_o)) {
-    for (; j < jend; j += LANES) {
-      const Epoch E = wsep[j - e0];
-      lc_sample<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is, lnt, s, flag);
+    if constexpr (!PIPE) {
+      for (; j < jend; j += LANES) {
+        const Epoch E = wsep[j - e0];
+        lc_accum<T, ONE>(lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is), R, lnt, s, flag);
+      }
+    } else if (j < jend) {
+      SampleTaps<T> P, Q;
+      {
+        const Epoch E = wsep[j - e0];
+        P = lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is);
+      }
+      while (true) {
+        if (j + LANES < jend) {
+          const Epoch E = wsep[j + LANES - e0];
+          Q = lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is);
+        }
+        lc_accum<T, ONE>(P, R, lnt, s, flag);
+        j += LANES;
+        if (j >= jend) break;
+        if (j + LANES < jend) {
+          const Epoch E = wsep[j + LANES - e0];
+          P = lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is);
+        }
+        lc_accum<T, ONE>(Q, R, lnt, s, flag);
+        j += LANES;
+        if (j >= jend) break;
+      }
     }
-  } else {
+  } else if constexpr (!PIPE) {
     Epoch en = j < jend ? ep[j] : Epoch{0.0, 0.0, 0.0, 0.0};
     for (; j < jend; j += LANES) {
       const Epoch E = en;
       if (j + LANES < jend) en = ep[j + LANES];
-      lc_sample<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is, lnt, s, flag);
+      lc_accum<T, ONE>(lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is), R, lnt, s, flag);
+    }
+  } else if (j < jend) {
+    // the record of the epoch after next is read one sample ahead of its
+    // tap loads
+    SampleTaps<T> P, Q;
+    Epoch en;
+    {
+      const Epoch E = ep[j];
+      P = lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, E.t, E.m, E.is);
+      if (j + LANES < jend) en = ep[j + LANES];
+    }
+    while (true) {
+      if (j + LANES < jend) {
+        Q = lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, en.t, en.m, en.is);
+        if (j + 2 * LANES < jend) en = ep[j + 2 * LANES];
+      }
+      lc_accum<T, ONE>(P, R, lnt, s, flag);
+      j += LANES;
+      if (j >= jend) break;
+      if (j + LANES < jend) {
+        P = lc_fetch<T, ONE>(nv, Nx, Ny, rowstep, bsc, R, en.t, en.m, en.is);
+        if (j + 2 * LANES < jend) en = ep[j + 2 * LANES];
+      }
+      lc_accum<T, ONE>(Q, R, lnt, s, flag);
+      j += LANES;
+      if (j >= jend) break;
     }
   }
 }
@@ -739,6 +849,10 @@ __global__ __launch_bounds__(256) void k_lc_batch(
     int* __restrict__ qctr, double* __restrict__ acc, int* __restrict__ bad, int bsel) {
   constexpr int GPW = 64 / LANES;
   constexpr bool kF32 = sizeof(T) == 4;
+  // one kernel holds the single- and multi-node paths, so one register
+  // count: FP32 keeps the unpipelined loops (111-121 VGPRs, four waves per
+  // SIMD, against 149-159 with the pipelined multi-node loop)
+  constexpr bool kBatchPipe = !kF32;
   __shared__ LnEntry lnt[kF32 ? kLnTab : 1];
   __shared__ Epoch sep[4][kStageEpochs];
   if constexpr (kF32) {
@@ -793,10 +907,10 @@ __global__ __launch_bounds__(256) void k_lc_batch(
           }
           const NodeView<JobNodes<T>> nv{cntn, J->rad[0], J->rad[cntn - 1], J->rad, J->ih, JobNodes<T>{J}};
           if (J->one)
-            lc_band_epochs<T, true, LANES>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off, nepoch,
+            lc_band_epochs<T, true, LANES, kBatchPipe>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off, nepoch,
                                            ep, wsep, st_o, lnt, s, flag);
           else
-            lc_band_epochs<T, false, LANES>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off,
+            lc_band_epochs<T, false, LANES, kBatchPipe>(nv, Nx, Ny, bsc, R, act, ok, e0, e1, b, B, nobs, obs_off,
                                             nepoch, ep, wsep, st_o, lnt, s, flag);
           const double val = group_sum8<LANES>(s, li);
           const unsigned long long bm = __ballot(flag);
