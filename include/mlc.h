@@ -297,11 +297,10 @@ int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes,
  *   mlc_map_convolve_nodes(map, nnodes, node_shape, node_rad, pix, NULL, NULL,
  *                          {d_nodes + k * Nx*Ny pairs})
  * followed by mlc_lc_chi2_grad(ctx of map, precision of map, d_nodes, ...),
- * with bit-identical node maps, chi2 and gradient, but node k is convolved
- * right before the first (profile, band) launch that reads it, so that
- * launch finds its maps in the last-level cache instead of HBM.  The order
- * mirrors the per-profile convolve-then-sample loop of expanding_source
- * (gpu_functions.cu:180-225).  d_nodes: nnodes consecutive node maps (the
+ * with bit-identical node maps, chi2 and gradient, in one call (it replaces
+ * the per-profile convolve-then-sample loop of expanding_source,
+ * gpu_functions.cu:180-225; every node map is written before the first
+ * band launch).  d_nodes: nnodes consecutive node maps (the
  * layout mlc_map_convolve writes to d_node); node_shape: host array of
  * nnodes MLC_SHAPE_* values; node_rad doubles as each node's half-light
  * radius (in units of pix) for the convolution.  Every node map is written,

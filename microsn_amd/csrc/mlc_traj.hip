@@ -1225,13 +1225,10 @@ LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(Launc
   return best;
 }
 
-// before_band(p*B + b), when given, runs on the host before the launch of
-// band b of profile p is queued (the caller writes that band's node maps).
 // With `fixed` the call's scratch comes from that buffer instead of the
 // context's workspace (no allocation; too small a buffer is an argument error
 // raised before any launch).
-void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* before_band = nullptr,
-               Workspace* fixed = nullptr) {
+void run_fused(mlc_ctx* ctx, const Args& a, Workspace* fixed = nullptr) {
   validate_fused(a);
   if (a.ncurves == 0) return;
   const int P = a.P, B = a.B, n = a.ncurves;
@@ -1280,7 +1277,7 @@ void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* befo
   // every (profile, band) launch with `lanes` lanes per curve and gmul times
   // as many workgroups as the CUs hold at once (a multiple of 8, so that
   // every XCD gets the same number)
-  auto launch_bands = [&](int lanes, int gmul, const std::function<void(int)>* before) {
+  auto launch_bands = [&](int lanes, int gmul) {
     with_lanes(lanes, [&](auto lc) {
       constexpr int L = decltype(lc)::value;
       auto grid_for = [&](const void* k) {
@@ -1296,7 +1293,6 @@ void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* befo
           bn.bsc = a.band_scale[b];
           for (int k = 0; k < bn.cnt; ++k) bn.rad[k] = a.node_rad[a.node_off[pb] + k];
           const void* base = (const char*)a.nodes + esz * mapsz * a.node_off[pb];
-          if (before) (*before)(pb);
 #define MLC_BAND(TT, F, O)                                                                          \
   MLC_LAUNCH(ctx, "lc_chi2_grad", (k_lc_band<TT, F, O, L>),                                         \
              dim3(grid_for(reinterpret_cast<const void*>(&k_lc_band<TT, F, O, L>))), dim3(256), 0,   \
@@ -1325,16 +1321,11 @@ void run_fused(mlc_ctx* ctx, const Args& a, const std::function<void(int)>* befo
   if (!fixed && needs_tuning(ctx, key)) {
     // (the allocation-free variant never tunes: it uses a kept shape or the
     // default)
-    // every node map must exist before the candidates run: the caller's
-    // per-band work (node convolutions) is done first, in band order
-    if (before_band)
-      for (int pb = 0; pb < P * B; ++pb) (*before_band)(pb);
-    sh = tune_pick(ctx, key, [&](LaunchShape c) { launch_bands(c.lanes, c.gmul, nullptr); });
-    before_band = nullptr;
+    sh = tune_pick(ctx, key, [&](LaunchShape c) { launch_bands(c.lanes, c.gmul); });
   } else {
     sh = tuned_shape(ctx, key);
   }
-  launch_bands(sh.lanes, sh.gmul, before_band);
+  launch_bands(sh.lanes, sh.gmul);
   MLC_LAUNCH(ctx, "lc_finish", k_lc_finish, dim3(cdiv(n, 256)), dim3(256), 0, n, P, start, pos_of,
              a.params, acc, bad, a.chi2, a.grad);
 }
@@ -1610,21 +1601,14 @@ int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix, v
                          (size_t)map->Nx * map->Ny;
     std::vector<void*> dst(nnodes);
     for (int k = 0; k < nnodes; ++k) dst[k] = (char*)d_nodes + bytes * k;
-    // node k is convolved just before the first band launch that reads it,
-    // so that launch finds it in the last-level cache; nodes no band reads
-    // (and all nodes when there are no curves) follow at the end
-    std::vector<char> done(nnodes, 0);
+    // every node map first, then the curves: the node kernels run back to
+    // back while the row spectrum they all read stays in the last-level
+    // cache (convolving each node right before the band launch that reads
+    // it let the band kernels evict the spectrum between nodes, and the step
+    // took 3% longer)
     NodeDriver drive = [&](const std::function<void(int)>& conv) {
-      std::function<void(int)> before = [&](int pb) {
-        for (int k = node_off[pb]; k < node_off[pb] + node_cnt[pb]; ++k)
-          if (!done[k]) {
-            conv(k);
-            done[k] = 1;
-          }
-      };
-      run_fused(ctx, a, &before);
-      for (int k = 0; k < nnodes; ++k)
-        if (!done[k]) conv(k);
+      for (int k = 0; k < nnodes; ++k) conv(k);
+      run_fused(ctx, a);
     };
     if (map->precision == MLC_F64) fft_convolve_nodes<double>(map, Ps, dst.data(), &drive);
     else fft_convolve_nodes<float>(map, Ps, dst.data(), &drive);
@@ -1642,7 +1626,7 @@ int mlc_lc_chi2_grad_noalloc(mlc_ctx* ctx, int precision, const void* d_nodes, i
            ncurves, d_params, d_prof, d_obs, nobs, d_obs_off, nepoch, d_t, d_mobs, d_sig, d_chi2,
            d_grad};
     Workspace ws(ctx, scratch_ptr(ctx), kNoAllocScratchBytes, "lc_chi2_grad_noalloc");
-    run_fused(ctx, a, nullptr, &ws);
+    run_fused(ctx, a, &ws);
   });
 }
 

@@ -114,9 +114,8 @@ def chi2_grad(ctx, nodes: NodeSet, obs: Observations, params, prof, obs_idx, pla
     """(chi2 [n], grad [n, 7]) FP64 device tensors for n curves.
 
     convolve=True first recomputes the node maps from the spectrum's current
-    map, in the same call (mlc_lc_chi2_grad_convolve: each node right before
-    the band launch that reads it); the result equals nodes.compute()
-    followed by chi2_grad(...) bit for bit."""
+    map, in the same call (mlc_lc_chi2_grad_convolve); the result equals
+    nodes.compute() followed by chi2_grad(...) bit for bit."""
     _dev(ctx, params, torch.float64, "params")
     _dev(ctx, prof, torch.int32, "prof")
     _dev(ctx, obs_idx, torch.int32, "obs")

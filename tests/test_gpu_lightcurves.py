@@ -226,8 +226,8 @@ def test_autograd_function(ctx):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_convolve_entry_matches_separate_calls(ctx, prec):
-    """mlc_lc_chi2_grad_convolve (node k convolved right before the band
-    launch that reads it) against mlc_map_convolve_nodes followed by
+    """mlc_lc_chi2_grad_convolve (node maps and chi2 in one call) against
+    mlc_map_convolve_nodes followed by
     mlc_lc_chi2_grad: node maps, chi2 and gradient bit for bit; with no
     curves every node map is still written."""
     rng = np.random.default_rng(77)
