@@ -321,15 +321,15 @@ def config4_workload(ctx, prec="f32", ntraj=1_000_000, seed=4004):
 
 
 def batch_step(ctx, w):
-    """One step of a batched workload: every map's spectrum and node maps,
-    the chi2 + adjoint of all curves in one batched call, the stable sort of
+    """One step of a batched workload: every map's spectrum and node maps
+    (one batched call each), the chi2 + adjoint of all curves in one batched
+    call, the stable sort of
     the chi2 values and their bin table, with no host synchronisation."""
     import microsn_amd.api as api
     import microsn_amd.lightcurves as lcm
 
-    for spec, m, ns in zip(w["specs"], w["maps"], w["nodesets"]):
-        spec.update(m)
-        ns.compute()
+    api.update_spectra(w["specs"], w["maps"])
+    w["batch"].compute()
     c, g = lcm.chi2_grad_batch(ctx, w["batch"], w["obs"], w["params"], w["map_idx"], w["prof"], w["obs_idx"])
     perm, _, sk = api.sort_f64(ctx, c, sorted_keys=True)
     lo, npb, bc, be = api.bin_table_device(ctx, sk, w["bins"])

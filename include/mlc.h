@@ -157,6 +157,20 @@ int mlc_map_convolve(mlc_map* map, int shape, double rhalf, double pix,
  * source sizes is the caller this serves.  Host arrays, device d_nodes[k]. */
 int mlc_map_convolve_nodes(mlc_map* map, int count, const int* shape, const double* rhalf,
                            double pix, const double* rin, const double* a, void* const* d_nodes);
+/* Several maps of one context and one precision in one call (many image
+ * systems or lens models of a sampler, each with its own map; the reference
+ * handles one map per run, gpu_functions.cu:151-175):
+ *   mlc_maps_update:  mlc_map_update(maps[i], d_maps[i]) for every i;
+ *   mlc_maps_convolve_nodes: mlc_map_convolve_nodes(maps[i], counts[i], ...)
+ *     for every i, with map i's nodes at offsets counts[0] + ... +
+ *     counts[i-1] of shape, rhalf, rin, a and d_nodes;
+ * bit for bit, but the small per-map kernels (the mean and DC terms of the
+ * updates; the profiles and cosine taps of the nodes) run once for all maps.
+ * Host arrays of nmaps (counts: of sum counts) entries. */
+int mlc_maps_update(int nmaps, mlc_map* const* maps, const double* const* d_maps);
+int mlc_maps_convolve_nodes(int nmaps, mlc_map* const* maps, const int* counts, const int* shape,
+                            const double* rhalf, double pix, const double* rin, const double* a,
+                            void* const* d_nodes);
 /* The same computation structured like the reference (plain path): build
  * the full Nx x Ny kernel array on the device, forward 2-D FFT, separate
  * multiply launch, inverse 2-D FFT, separate interleave, stream-synchronised

@@ -49,6 +49,8 @@ SIGNATURES = {
     "mlc_profile_size": (_i, [_i, _d, _d, _d, _d]),
     "mlc_map_convolve": (_i, [_p, _i, _d, _d, _d, _d, _p, _p]),
     "mlc_map_convolve_nodes": (_i, [_p, _i, _p, _p, _d, _p, _p, _p]),
+    "mlc_maps_update": (_i, [_i, _p, _p]),
+    "mlc_maps_convolve_nodes": (_i, [_i, _p, _p, _p, _p, _d, _p, _p, _p]),
     "mlc_map_convolve_plain": (_i, [_p, _i, _d, _d, _d, _d, _p, _p]),
     "mlc_fft_r2c": (_i, [_p, _i, _p, _p, _i, _i]),
     "mlc_fft_c2r": (_i, [_p, _i, _p, _p, _i, _i]),

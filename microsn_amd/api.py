@@ -204,6 +204,28 @@ class MapSpectrum:
             pass
 
 
+def update_spectra(spectra, maps):
+    """spectra[i].update(maps[i]) for every i in one call (mlc_maps_update:
+    the same bits, the small per-map kernels run once for all maps).  The
+    spectra share one context and precision."""
+    spectra, maps = list(spectra), list(maps)
+    if len(spectra) != len(maps):
+        raise ValueError("update_spectra: one map per spectrum")
+    if not spectra:
+        return
+    ctx = spectra[0].ctx
+    for sp, m in zip(spectra, maps):
+        if sp.ctx is not ctx:
+            raise ValueError("update_spectra: spectra of different contexts")
+        _dev(ctx, m, torch.float64, "map")
+        if tuple(m.shape) != (sp.Ny, sp.Nx):
+            raise ValueError(f"update_spectra: map of shape {tuple(m.shape)} for a {sp.Ny} x {sp.Nx} spectrum")
+    n = len(spectra)
+    hs = (C.c_void_p * n)(*[sp.h.value for sp in spectra])
+    ds = (C.c_void_p * n)(*[m.data_ptr() for m in maps])
+    ctx.check(lib().mlc_maps_update(n, hs, ds))
+
+
 def untile(t):
     """Row-major (..., Ny, Nx, 2) copy of a node map stored in 4 x 4 pixel tiles."""
     *lead, Ny, Nx, two = t.shape

@@ -18,7 +18,7 @@ ProfileParams make_params(int shape, double rhalf, double pix, double rin, doubl
   MLC_REQUIRE(std::isfinite(pix) && pix > 0, "pixel size must be positive and finite");
   if (shape == MLC_SHAPE_GAUSSIAN_HOLE) MLC_REQUIRE(rin >= 0 && rin < rhalf, "need 0 <= Rin < rhalf");
   if (shape == MLC_SHAPE_WAVY) MLC_REQUIRE(a != 0.0, "wavy profile needs a != 0");
-  ProfileParams P;
+  ProfileParams P{};  // every byte defined: node tables compare their bytes
   MLC_REQUIRE(profile_params(shape, rhalf, pix, rin, a, &P), "profile has an empty grid");
   if (Nx > 0)
     MLC_REQUIRE(P.n <= Nx && P.n <= Ny, "profile grid (" + std::to_string(P.n) +
@@ -129,6 +129,60 @@ int mlc_map_update(mlc_map* m, const double* d_map) {
     MLC_REQUIRE(d_map, "mlc_map_update: null map");
     if (m->precision == MLC_F64) fft_map_transform<double>(m, d_map);
     else fft_map_transform<float>(m, d_map);
+  });
+}
+
+namespace {
+// The maps of a batched call: non-null, one context, one precision.
+void check_maps(int nmaps, mlc_map* const* maps, const char* what) {
+  MLC_REQUIRE(maps, std::string(what) + ": null map list");
+  for (int i = 0; i < nmaps; ++i) {
+    MLC_REQUIRE(maps[i], std::string(what) + ": null map");
+    MLC_REQUIRE(maps[i]->ctx == maps[0]->ctx, std::string(what) + ": maps of different contexts");
+    MLC_REQUIRE(maps[i]->precision == maps[0]->precision, std::string(what) + ": maps of different precisions");
+  }
+}
+}  // namespace
+
+int mlc_maps_update(int nmaps, mlc_map* const* maps, const double* const* d_maps) {
+  if (nmaps < 0 || (nmaps > 0 && (!maps || !maps[0]))) return MLC_ERR_ARG;
+  if (nmaps == 0) return MLC_OK;
+  return guarded(maps[0]->ctx, [&] {
+    check_maps(nmaps, maps, "mlc_maps_update");
+    MLC_REQUIRE(d_maps, "mlc_maps_update: null source list");
+    for (int i = 0; i < nmaps; ++i) MLC_REQUIRE(d_maps[i], "mlc_maps_update: null map");
+    std::vector<mlc_map*> ms(maps, maps + nmaps);
+    if (maps[0]->precision == MLC_F64) fft_maps_transform<double>(ms, d_maps);
+    else fft_maps_transform<float>(ms, d_maps);
+  });
+}
+
+int mlc_maps_convolve_nodes(int nmaps, mlc_map* const* maps, const int* counts, const int* shape,
+                            const double* rhalf, double pix, const double* rin, const double* a,
+                            void* const* d_nodes) {
+  if (nmaps < 0 || (nmaps > 0 && (!maps || !maps[0]))) return MLC_ERR_ARG;
+  if (nmaps == 0) return MLC_OK;
+  return guarded(maps[0]->ctx, [&] {
+    check_maps(nmaps, maps, "mlc_maps_convolve_nodes");
+    MLC_REQUIRE(counts, "mlc_maps_convolve_nodes: null counts");
+    std::vector<mlc_map*> ms(maps, maps + nmaps);
+    std::vector<std::vector<ProfileParams>> Ps(nmaps);
+    int off = 0;
+    for (int i = 0; i < nmaps; ++i) {
+      MLC_REQUIRE(counts[i] >= 0, "mlc_maps_convolve_nodes: negative count");
+      if (counts[i] == 0) continue;
+      MLC_REQUIRE(shape && rhalf && d_nodes, "mlc_maps_convolve_nodes: null pointer");
+      MLC_REQUIRE(maps[i]->Nx % 4 == 0 && maps[i]->Ny % 4 == 0,
+                  "mlc_maps_convolve_nodes: node maps (4 x 4 tiles) need Nx and Ny divisible by 4");
+      for (int k = off; k < off + counts[i]; ++k) {
+        MLC_REQUIRE(d_nodes[k], "mlc_maps_convolve_nodes: null node map");
+        Ps[i].push_back(make_params(shape[k], rhalf[k], pix, rin ? rin[k] : 0.0, a ? a[k] : 1.0, maps[i]->Nx,
+                                    maps[i]->Ny));
+      }
+      off += counts[i];
+    }
+    if (maps[0]->precision == MLC_F64) fft_maps_convolve_nodes<double>(ms, Ps, d_nodes);
+    else fft_maps_convolve_nodes<float>(ms, Ps, d_nodes);
   });
 }
 

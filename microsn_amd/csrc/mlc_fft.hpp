@@ -50,6 +50,15 @@ using NodeDriver = std::function<void(const std::function<void(int)>& conv)>;
 template <class T>
 void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes,
                         const NodeDriver* drive = nullptr);
+// Several maps of one context (see mlc_maps_update, mlc_maps_convolve_nodes):
+// the same kernels and sums as the per-map calls, with the small per-map
+// kernels of all maps in one launch each.  Ps[i] are the nodes of map i;
+// d_nodes lists the node maps of map 0, then map 1, ...
+template <class T>
+void fft_maps_transform(const std::vector<mlc_map*>& maps, const double* const* d_maps);
+template <class T>
+void fft_maps_convolve_nodes(const std::vector<mlc_map*>& maps, const std::vector<std::vector<ProfileParams>>& Ps,
+                             void* const* d_nodes);
 // Reference-structured convolution (see mlc_map_convolve_plain).
 template <class T>
 void fft_convolve_plain(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
@@ -67,6 +76,12 @@ extern template void fft_convolve_nodes<float>(mlc_map*, const std::vector<Profi
                                                const NodeDriver*);
 extern template void fft_convolve_nodes<double>(mlc_map*, const std::vector<ProfileParams>&, void* const*,
                                                 const NodeDriver*);
+extern template void fft_maps_transform<float>(const std::vector<mlc_map*>&, const double* const*);
+extern template void fft_maps_transform<double>(const std::vector<mlc_map*>&, const double* const*);
+extern template void fft_maps_convolve_nodes<float>(const std::vector<mlc_map*>&,
+                                                    const std::vector<std::vector<ProfileParams>>&, void* const*);
+extern template void fft_maps_convolve_nodes<double>(const std::vector<mlc_map*>&,
+                                                     const std::vector<std::vector<ProfileParams>>&, void* const*);
 extern template void fft_convolve_plain<float>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_convolve_plain<double>(mlc_map*, const ProfileParams&, void*, void*);
 extern template void fft_r2c_2d<float>(mlc_ctx*, const float*, cpx<float>*, int, int);

@@ -23,6 +23,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -220,9 +221,8 @@ __global__ __launch_bounds__(G) void k_row_r2c(const TIN* __restrict__ in, cpx<T
 // fixed order / (Nx Ny)) into *mean and the DC term of every row spectrum,
 // rowsum[y] - Nx mean, the FFT of the row minus the map mean would give.
 template <class T>
-__global__ __launch_bounds__(1024) void k_row_dc_fix(const double* __restrict__ rowsum, int Nx, int Ny,
-                                                     cpx<T>* __restrict__ out, int ldo,
-                                                     double* __restrict__ mean) {
+__device__ __forceinline__ void row_dc_fix_block(const double* __restrict__ rowsum, int Nx, int Ny,
+                                                 cpx<T>* __restrict__ out, int ldo, double* __restrict__ mean) {
   __shared__ double red[1024];
   const int t = threadIdx.x;
   double a = 0.0;
@@ -236,6 +236,26 @@ __global__ __launch_bounds__(1024) void k_row_dc_fix(const double* __restrict__ 
   const double mu = red[0] / ((double)Nx * (double)Ny);
   if (t == 0) *mean = mu;
   for (int y = t; y < Ny; y += 1024) out[(size_t)y * ldo] = mk<T>((T)(rowsum[y] - (double)Nx * mu), (T)0);
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_row_dc_fix(const double* __restrict__ rowsum, int Nx, int Ny,
+                                                     cpx<T>* __restrict__ out, int ldo,
+                                                     double* __restrict__ mean) {
+  row_dc_fix_block<T>(rowsum, Nx, Ny, out, ldo, mean);
+}
+
+// The DC step of several maps, one workgroup per map (the batched update).
+struct DcJob {
+  const double* rowsum;
+  void* out;
+  double* mean;
+  int Nx, Ny, ldo;
+};
+template <class T>
+__global__ __launch_bounds__(1024) void k_row_dc_fix_many(const DcJob* __restrict__ jobs) {
+  const DcJob& J = jobs[blockIdx.x];
+  row_dc_fix_block<T>(J.rowsum, J.Nx, J.Ny, static_cast<cpx<T>*>(J.out), J.ldo, J.mean);
 }
 
 // Loads the folded inverse input of one row into buf (see c2r_pre).
@@ -976,20 +996,23 @@ __global__ __launch_bounds__(1024) void k_profile(ProfileParams P, double* __res
   profile_block(P, prof, dprof);
 }
 
-// Several node maps in one call: profile k at prof + off_p, its cosine
-// coefficients at av / ad + off_a (offsets in doubles) and its taps in the
-// working precision at tp + off_t (in complex elements).
+// One node map of a call (the nodes of one map, or of several maps of the
+// batched call): its profile grid and d/drhalf (n x n each), its cosine
+// coefficients (h x W each, FP64) and its taps in the working precision
+// (hq x ldt complex elements), for a map of width Nx with W = Nx/2 + 1 bins
+// and the cosine table cosx of that width.
 struct NodeJob {
   ProfileParams P;
-  int64_t off_p, off_a, off_t;
+  double *prof, *dprof, *av, *ad;
+  void* tp;
+  const double* cosx;
+  int Nx, W, ldt;
 };
 
 // One workgroup per job.
-__global__ __launch_bounds__(1024) void k_profile_many(const NodeJob* __restrict__ jobs,
-                                                       double* __restrict__ prof,
-                                                       double* __restrict__ dprof) {
-  const NodeJob J = jobs[blockIdx.x];
-  profile_block(J.P, prof + J.off_p, dprof + J.off_p);
+__global__ __launch_bounds__(1024) void k_profile_many(const NodeJob* __restrict__ jobs) {
+  const NodeJob& J = jobs[blockIdx.x];
+  profile_block(J.P, J.prof, J.dprof);
 }
 
 // Cosine coefficients of the symmetric kernel (see k_node_col):
@@ -1036,24 +1059,18 @@ __global__ __launch_bounds__(256) void k_kernel_cos(const double* __restrict__ p
   kernel_cos_at<T>(prof, dprof, n, Nx, W, cosx, av, ad, tp, ldt, u, k);
 }
 
-// grid (ceil(W/256), max h rounded up to kNfK, jobs).
+// grid (ceil(max W / 256), max h rounded up to kNfK, jobs).
 template <class T>
-__global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restrict__ jobs,
-                                                         const double* __restrict__ prof,
-                                                         const double* __restrict__ dprof, int Nx,
-                                                         int W, const double* __restrict__ cosx,
-                                                         double* __restrict__ av,
-                                                         double* __restrict__ ad,
-                                                         cpx<T>* __restrict__ tp, int ldt) {
+__global__ __launch_bounds__(256) void k_kernel_cos_many(const NodeJob* __restrict__ jobs) {
   const NodeJob& J = jobs[blockIdx.z];
   const int n = J.P.n, u = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, h = n >> 1;
-  if (u >= W) return;
+  if (u >= J.W) return;
+  cpx<T>* tp = static_cast<cpx<T>*>(J.tp);
   if (k >= h) {  // the zero taps up to a multiple of kNfK
-    if (k < (h + kNfK - 1) / kNfK * kNfK) tp[J.off_t + (size_t)k * ldt + u] = mk<T>(0, 0);
+    if (k < (h + kNfK - 1) / kNfK * kNfK) tp[(size_t)k * J.ldt + u] = mk<T>(0, 0);
     return;
   }
-  kernel_cos_at<T>(prof + J.off_p, dprof + J.off_p, n, Nx, W, cosx, av + J.off_a, ad + J.off_a,
-                   tp + J.off_t, ldt, u, k);
+  kernel_cos_at<T>(J.prof, J.dprof, n, J.Nx, J.W, J.cosx, J.av, J.ad, tp, J.ldt, u, k);
 }
 
 // Full Ny x Nx wrapped kernel (Kernel::setKernel), for the plain path.
@@ -1525,6 +1542,44 @@ bool node_yfilter(mlc_ctx* ctx, const cpx<T>* rows, int ldg, const double* av, c
 namespace mlc {
 
 template <class T>
+void fft_maps_transform(const std::vector<mlc_map*>& maps, const double* const* d_maps) {
+  mlc_ctx* ctx = maps[0]->ctx;
+  const int nm = (int)maps.size();
+  Workspace ws(ctx);
+  std::vector<int> ir(nm, -1);
+  for (int i = 0; i < nm; ++i)
+    if (maps[i]->subtract_mean) ir[i] = ws.add(sizeof(double) * maps[i]->Ny);
+  ws.commit();
+  // every row transform, then the mean and DC step of all mean-subtracting
+  // maps in one launch (the same sums as fft_map_transform's)
+  std::vector<DcJob> dc;
+  for (int i = 0; i < nm; ++i) {
+    mlc_map* m = maps[i];
+    if (m->subtract_mean) {
+      double* rowsum = ws.get<double>(ir[i]);
+      if (!map_rows<T>(ctx, d_maps[i], (cpx<T>*)m->rows, m->ldr, rowsum, m->Nx, m->Ny))
+        row_r2c<T, double>(ctx, d_maps[i], (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, rowsum, m->ldr);
+      DcJob J;
+      std::memset(&J, 0, sizeof J);  // (padding too: see node_tables)
+      J.rowsum = rowsum;
+      J.out = m->rows;
+      J.mean = m->mean;
+      J.Nx = m->Nx;
+      J.Ny = m->Ny;
+      J.ldo = m->ldr;
+      dc.push_back(J);
+    } else {
+      row_r2c<T, double>(ctx, d_maps[i], (cpx<T>*)m->rows, m->Nx, m->Ny, nullptr, nullptr, m->ldr);
+    }
+    m->full_ready = false;
+  }
+  if (!dc.empty()) {
+    const DcJob* dj = static_cast<const DcJob*>(const_table(ctx, kTableDcJobs, dc.data(), sizeof(DcJob) * dc.size()));
+    MLC_LAUNCH(ctx, "map_dc", k_row_dc_fix_many<T>, dim3((int)dc.size()), dim3(1024), 0, dj);
+  }
+}
+
+template <class T>
 void fft_map_transform(mlc_map* m, const double* d_map) {
   mlc_ctx* ctx = m->ctx;
   if (m->subtract_mean) {
@@ -1601,62 +1656,119 @@ void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void*
 // order or in the order a driver asks for them.  The profile tables live in
 // the map's own buffer, so a driver may run work that uses the context
 // workspace between the nodes.
+// Profile grids, cosine coefficients and taps of the nodes Ps of map m, in
+// m's node-table storage (one slice each): one NodeJob per node, appended to
+// jobs; hmax and wmax grow to the largest tap rows (a multiple of kNfK) and
+// the widest bin count.
+template <class T>
+void node_tables(mlc_map* m, const std::vector<ProfileParams>& Ps, std::vector<NodeJob>& jobs, int& hmax,
+                 int& wmax) {
+  mlc_ctx* ctx = m->ctx;
+  const int cnt = (int)Ps.size(), W = m->W, ldt = W + (W & 1);
+  Workspace ws(ctx, &m->ntab, &m->ntab_bytes);
+  std::vector<int> sl(5 * (size_t)cnt);
+  for (int k = 0; k < cnt; ++k) {
+    const int n = Ps[k].n, h = n / 2, hq = (h + kNfK - 1) / kNfK * kNfK;
+    sl[5 * k] = ws.add(sizeof(double) * n * n);
+    sl[5 * k + 1] = ws.add(sizeof(double) * n * n);
+    sl[5 * k + 2] = ws.add(sizeof(double) * std::max<int64_t>((int64_t)h * W, 1));
+    sl[5 * k + 3] = ws.add(sizeof(double) * std::max<int64_t>((int64_t)h * W, 1));
+    sl[5 * k + 4] = ws.add(sizeof(cpx<T>) * std::max<int64_t>((int64_t)hq * ldt, 1));
+    hmax = std::max(hmax, hq);
+  }
+  wmax = std::max(wmax, W);
+  ws.commit();
+  const double* cosx = cos_table(ctx, 1, m->Nx);
+  for (int k = 0; k < cnt; ++k) {
+    // zeroed first: the device table is rewritten only when its bytes,
+    // padding included, change
+    NodeJob J;
+    std::memset(&J, 0, sizeof J);
+    J.P = Ps[k];
+    J.prof = ws.get<double>(sl[5 * k]);
+    J.dprof = ws.get<double>(sl[5 * k + 1]);
+    J.av = ws.get<double>(sl[5 * k + 2]);
+    J.ad = ws.get<double>(sl[5 * k + 3]);
+    J.tp = ws.get<cpx<T>>(sl[5 * k + 4]);
+    J.cosx = cosx;
+    J.Nx = m->Nx;
+    J.W = W;
+    J.ldt = ldt;
+    jobs.push_back(J);
+  }
+}
+
+// The profiles and taps of every job in the device table dj (njobs jobs).
+template <class T>
+void launch_node_tables(mlc_ctx* ctx, const NodeJob* dj, int njobs, int hmax, int wmax) {
+  MLC_LAUNCH(ctx, "profile", k_profile_many, dim3(njobs), dim3(1024), 0, dj);
+  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many<T>, dim3(cdiv(wmax, 256), hmax, njobs), dim3(256), 0,
+             dj);
+}
+
+// Node map J of map m into dst: the fused kernel, else the separate y-filter
+// (or the column-FFT path) and row kernels.
+template <class T>
+void convolve_node(mlc_map* m, const NodeJob& J, void* dst) {
+  mlc_ctx* ctx = m->ctx;
+  const int h = J.P.n / 2, W = m->W;
+  if (node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, static_cast<const cpx<T>*>(J.tp), J.ldt, h, (T*)dst,
+                    m->Nx, m->Ny, m->mean))
+    return;
+  cpx<T>* G0 = (cpx<T>*)m->g0;
+  cpx<T>* G1 = (cpx<T>*)m->g1;
+  if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, m->ldr, J.av, J.ad, h, G0, G1, m->Nx, m->Ny, W)) {
+    ensure_full<T>(m);
+    node_col<T>(ctx, (const cpx<T>*)m->fmap, J.av, J.ad, h, G0, G1, m->Nx, m->Ny, W);
+  }
+  if (!node_rows<T>(ctx, G0, G1, (T*)dst, m->Nx, m->Ny, m->mean))
+    row_c2r<T, 2>(ctx, G0, G1, (T*)dst, m->Nx, m->Ny, m->mean);
+}
+
 template <class T>
 void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes,
                         const NodeDriver* drive) {
   mlc_ctx* ctx = m->ctx;
-  const int cnt = (int)Ps.size(), W = m->W;
+  const int cnt = (int)Ps.size();
   if (cnt == 0) {
     if (drive) (*drive)([](int) {});
     return;
   }
-  std::vector<NodeJob> jobs(cnt);
-  const int ldt = W + (W & 1);
-  int64_t np = 0, na = 0, nt = 0;
-  int hmax = 1;
-  for (int k = 0; k < cnt; ++k) {
-    const int n = Ps[k].n, h = n / 2;
-    const int hq = (h + kNfK - 1) / kNfK * kNfK;
-    jobs[k] = NodeJob{Ps[k], np, na, nt};
-    np += (int64_t)n * n;
-    na += (int64_t)h * W;
-    nt += (int64_t)hq * ldt;
-    hmax = std::max(hmax, hq);
-  }
-  Workspace ws(ctx, &m->ntab, &m->ntab_bytes);
-  int ip = ws.add(sizeof(double) * np), idp = ws.add(sizeof(double) * np);
-  int ia = ws.add(sizeof(double) * std::max<int64_t>(na, 1)), iad = ws.add(sizeof(double) * std::max<int64_t>(na, 1));
-  int itp = ws.add(sizeof(cpx<T>) * std::max<int64_t>(nt, 1));
-  ws.commit();
+  std::vector<NodeJob> jobs;
+  int hmax = 1, wmax = 1;
+  node_tables<T>(m, Ps, jobs, hmax, wmax);
   const NodeJob* dj =
       static_cast<const NodeJob*>(const_table_in(ctx, m->jobs, jobs.data(), sizeof(NodeJob) * cnt));
-  double *prof = ws.get<double>(ip), *dprof = ws.get<double>(idp);
-  double *av = ws.get<double>(ia), *ad = ws.get<double>(iad);
-  cpx<T>* tp = ws.get<cpx<T>>(itp);
-  MLC_LAUNCH(ctx, "profile", k_profile_many, dim3(cnt), dim3(1024), 0, dj, prof, dprof);
-  MLC_LAUNCH(ctx, "conv_kernel_cos", k_kernel_cos_many<T>, dim3(cdiv(W, 256), hmax, cnt), dim3(256), 0, dj,
-             prof, dprof, m->Nx, W, cos_table(ctx, 1, m->Nx), av, ad, tp, ldt);
-  cpx<T>* G0 = (cpx<T>*)m->g0;
-  cpx<T>* G1 = (cpx<T>*)m->g1;
+  launch_node_tables<T>(ctx, dj, cnt, hmax, wmax);
   auto conv = [&](int k) {
     MLC_REQUIRE(k >= 0 && k < cnt, "convolve_nodes: node index out of range");
-    const int h = Ps[k].n / 2;
-    if (node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, tp + jobs[k].off_t, ldt, h, (T*)d_nodes[k], m->Nx,
-                      m->Ny, m->mean))
-      return;
-    const double *a = av + jobs[k].off_a, *b = ad + jobs[k].off_a;
-    if (!node_yfilter<T>(ctx, (const cpx<T>*)m->rows, m->ldr, a, b, h, G0, G1, m->Nx, m->Ny, W)) {
-      ensure_full<T>(m);
-      node_col<T>(ctx, (const cpx<T>*)m->fmap, a, b, h, G0, G1, m->Nx, m->Ny, W);
-    }
-    if (!node_rows<T>(ctx, G0, G1, (T*)d_nodes[k], m->Nx, m->Ny, m->mean))
-      row_c2r<T, 2>(ctx, G0, G1, (T*)d_nodes[k], m->Nx, m->Ny, m->mean);
+    convolve_node<T>(m, jobs[k], d_nodes[k]);
   };
   if (drive) {
     (*drive)(conv);
   } else {
     for (int k = 0; k < cnt; ++k) conv(k);
   }
+}
+
+template <class T>
+void fft_maps_convolve_nodes(const std::vector<mlc_map*>& maps, const std::vector<std::vector<ProfileParams>>& Ps,
+                             void* const* d_nodes) {
+  mlc_ctx* ctx = maps[0]->ctx;
+  std::vector<NodeJob> jobs;
+  std::vector<int> first;  // first job of each map
+  int hmax = 1, wmax = 1;
+  for (size_t i = 0; i < maps.size(); ++i) {
+    first.push_back((int)jobs.size());
+    if (!Ps[i].empty()) node_tables<T>(maps[i], Ps[i], jobs, hmax, wmax);
+  }
+  if (jobs.empty()) return;
+  const NodeJob* dj =
+      static_cast<const NodeJob*>(const_table(ctx, kTableNodeJobs, jobs.data(), sizeof(NodeJob) * jobs.size()));
+  launch_node_tables<T>(ctx, dj, (int)jobs.size(), hmax, wmax);
+  for (size_t i = 0; i < maps.size(); ++i)
+    for (size_t k = 0; k < Ps[i].size(); ++k)
+      convolve_node<T>(maps[i], jobs[first[i] + k], d_nodes[first[i] + k]);
 }
 
 template <class T>
@@ -1718,6 +1830,10 @@ void fft_c2r_2d(mlc_ctx* ctx, cpx<T>* work, T* out, int Nx, int Ny) {
 
 #define MLC_FFT_INSTANTIATE(T)                                                               \
   template void fft_map_transform<T>(mlc_map*, const double*);                               \
+  template void fft_maps_transform<T>(const std::vector<mlc_map*>&, const double* const*);   \
+  template void fft_maps_convolve_nodes<T>(const std::vector<mlc_map*>&,                     \
+                                           const std::vector<std::vector<ProfileParams>>&,   \
+                                           void* const*);                                    \
   template void fft_convolve_fused<T>(mlc_map*, const ProfileParams&, void*, void*);         \
   template void fft_convolve_nodes<T>(mlc_map*, const std::vector<ProfileParams>&, void* const*,    \
                                       const NodeDriver*);                                    \

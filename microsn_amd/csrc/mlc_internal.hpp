@@ -219,7 +219,8 @@ int fill_grid(mlc_ctx* ctx, int per_cu);
 // uploaded when its bytes differ from the previous call's, reused otherwise,
 // so that a repeated call issues no host-to-device copy (and can be captured
 // into a graph once it has run).
-enum ConstSlot { kTableBins = 0, kTableNodeJobs = 1, kTableBatchJobs = 2, kTableBatchMaps = 3 };
+// kTableNodeJobs and kTableDcJobs: the batched map calls (mlc_maps_*)
+enum ConstSlot { kTableBins = 0, kTableNodeJobs = 1, kTableBatchJobs = 2, kTableBatchMaps = 3, kTableDcJobs = 4 };
 const void* const_table(mlc_ctx* ctx, int slot, const void* host, size_t bytes);
 // The same for a table owned by another object (e.g. a map's node jobs, so
 // that several maps used in one step each keep theirs on the device).

@@ -173,8 +173,26 @@ class NodeBatch:
         self.band_scale = np.ascontiguousarray(first.band_scale, np.float64)
 
     def compute(self):
-        for ns in self.nodesets:
-            ns.compute()
+        """(Re)convolve every node map of every map from its spectrum's
+        current map: one mlc_maps_convolve_nodes call (the same bits as
+        NodeSet.compute() on each set); per set when their pixel sizes
+        differ."""
+        sets = self.nodesets
+        if len({ns.pix for ns in sets}) > 1:
+            for ns in sets:
+                ns.compute()
+            return
+        if not hasattr(self, "_conv"):
+            nm = len(sets)
+            hs = (C.c_void_p * nm)(*[ns.spectrum.h.value for ns in sets])
+            counts = np.array([len(ns.node_rad) for ns in sets], np.int32)
+            shp = np.ascontiguousarray(np.concatenate([np.asarray(ns.node_shape, np.int32) for ns in sets]))
+            rad = np.ascontiguousarray(np.concatenate([ns.node_rad for ns in sets]), np.float64)
+            dst = (C.c_void_p * len(rad))(*[ns.nodes[k].data_ptr() for ns in sets for k in range(ns.nodes.shape[0])])
+            self._conv = (nm, hs, counts, shp, rad, dst)
+        nm, hs, counts, shp, rad, dst = self._conv
+        sets[0].ctx.check(lib().mlc_maps_convolve_nodes(nm, hs, counts.ctypes.data, shp.ctypes.data, rad.ctypes.data,
+                                                        sets[0].pix, None, None, dst))
 
 
 def chi2_grad_batch(ctx, batch: NodeBatch, obs: Observations, params, map_idx, prof, obs_idx):
