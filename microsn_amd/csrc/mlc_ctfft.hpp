@@ -56,12 +56,20 @@ struct CtStage {
   static constexpr int LNS = ct_log2(NS);
   using Regs = cpx<T>[BPT][R];
 
+  // pad16(j + r NB) = pad16(j) + r (NB + NB/16) and pad16(base + q NS) =
+  // pad16(base) + q (NS + NS/16) when 16 divides NB and NS (NS = 1: base is
+  // a multiple of 16 and q < 16), so the padded offsets are constants the
+  // LDS instructions carry and only pad16(j), pad16(base) are computed.
+  static_assert(NB % 16 == 0 && (NS == 1 || NS % 16 == 0), "padded strides are constants");
+  static constexpr int PNB = NB + NB / 16, PNS = NS == 1 ? 1 : NS + NS / 16;
+
   MLC_HD static void load(const cpx<T>* buf, const cpx<T>* lo, const cpx<T>* hi, int t, Regs& v) {
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
       const int j = t + b * Gm::G;
+      const cpx<T>* src = buf + pad16(j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(j + r * NB)];
+      for (int r = 0; r < R; ++r) v[b][r] = src[r * PNB];
       if constexpr (NS == 16) {
         const cpx<T>* t2 = hi + Gm::NHI + (j & 15);
 #pragma unroll
@@ -84,8 +92,9 @@ struct CtStage {
     for (int b = 0; b < BPT; ++b) {
       const int j = t + b * Gm::G;
       const int base = ((j >> LNS) << (LNS + ct_log2(R))) + (j & (NS - 1));
+      cpx<T>* dst = buf + pad16(base);
 #pragma unroll
-      for (int q = 0; q < R; ++q) buf[pad16(base + q * NS)] = v[b][q];
+      for (int q = 0; q < R; ++q) dst[q * PNS] = v[b][q];
     }
   }
 };
