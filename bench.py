@@ -82,9 +82,10 @@ def algorithmic_bytes(name, esz, ncurves):
     """Bytes one launch must move at least (value bytes of its operands).
 
     A chi2 launch covers one (profile, band): the curves of that profile,
-    one band's epochs each; every sample reads 4 bilinear taps of (mu,
-    dmu/dr) (8 esz bytes), every curve its 64-byte record and its 8 FP64
-    accumulators (read and written)."""
+    one band's epochs each.  It must read its node map of (mu, dmu/dr) once
+    and every curve's 64-byte record and 8 FP64 accumulators (read and
+    written); the bilinear taps its samples gather (tap_bytes) come through
+    the caches, most of them many times over."""
     W = N // 2 + 1
     spec = N * W * 2 * esz          # one half spectrum
     real = N * N * esz              # one real map
@@ -95,7 +96,7 @@ def algorithmic_bytes(name, esz, ncurves):
         "conv_node_fused": spec + 2 * real,            # read row spectrum, write (mu, dmu/dr)
         "fft_row_c2r_node": 2 * spec + 2 * real,       # read 2 spectra, write (mu, dmu/dr)
         "fft_row_r2c": N * N * 8 + spec,               # read FP64 map, write spectrum
-        "lc_chi2_grad": curves * EPOCHS * 4 * 2 * esz + curves * (64 + 2 * 64),
+        "lc_chi2_grad": 2 * real + curves * (64 + 2 * 64),
         # the FP64 plain path's stages (esz = 8)
         "fft_col_fwd": 2 * spec,                       # read + write the spectrum, in place
         "fft_col_inv": 2 * spec,
@@ -107,6 +108,12 @@ def algorithmic_bytes(name, esz, ncurves):
         "lc_plain_samples": ncurves * EPOCHS * len(BAND_SCALE) * (4 * 2 * esz + 32),
         "lc_plain_reduce": ncurves * EPOCHS * len(BAND_SCALE) * (32 + 24) + ncurves * 64,
     }.get(name)
+
+
+def tap_bytes(esz, ncurves):
+    """Bytes of bilinear taps one chi2 launch gathers: 4 (mu, dmu/dr) taps per
+    sample (L2/L1 traffic, not HBM)."""
+    return ncurves // len(SHAPES) * EPOCHS * 4 * 2 * esz
 
 
 def measured_traffic(kernel):
@@ -392,9 +399,12 @@ def time_batch_workload(ctx, which, steps):
     B = w["obs"].B
     samples = int((off[oi * B + B] - off[oi * B]).sum())
     ms, launches = kern["lc_batch"]
-    # per step: FP32 (mu, dmu/dr) taps; record read, accumulator written (per
-    # launch of the per-band form: its share)
-    ab = samples * 4 * 8 + ncurves * (64 + 64)
+    # per step: every node map read once, every record read and accumulator
+    # written (per launch of the per-band form: its share); the gathered
+    # FP32 (mu, dmu/dr) taps come through the caches
+    node_bytes = sum(ns.nodes.numel() * ns.nodes.element_size() for ns in w["nodesets"])
+    ab = node_bytes + ncurves * (64 + 64)
+    taps = samples * 4 * 8
     dom = max(kern, key=lambda k: kern[k][0])
     max_pix = max(int(m.shape[0]) * int(m.shape[1]) for m in w["maps"])
     lanes, gmul, flags = ctx.tune_lookup(ctx.TUNE_KIND_BATCH, "f32", w["obs"].nepoch / (w["obs"].nobs * B),
@@ -408,7 +418,8 @@ def time_batch_workload(ctx, which, steps):
                               "launches_per_step": launches, "one_launch_per_band": bool(flags & ctx.TUNE_BATCH_PER_BAND)},
            "lc_batch_roofline": {"bound": "hbm", "achieved": ab / (ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
                                  "unit": "GB/s", "frac": ab / (ms / 1e3) / HBM_PEAK,
-                                 "algorithmic_bytes_per_launch": ab / launches, "avg_launch_ms": ms / launches}}
+                                 "algorithmic_bytes_per_launch": ab / launches, "avg_launch_ms": ms / launches,
+                                 "tap_bytes_per_step": taps, "tap_rate_GBps": taps / (ms / 1e3) / 1e9}}
     for sp in w["specs"]:
         sp.close()
     del w, eager, out, g
@@ -817,12 +828,16 @@ def main():
         avg_s = ms_tot / launches / 1e3
         ab = algorithmic_bytes(k, 4, ncurves)
         tr = measured_traffic(k)
-        return {"kernel": k, "bound": "hbm", "achieved": ab / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": tr,
-                # the same launch time against the HBM bytes the counters measured
-                "frac_of_measured_traffic": tr / avg_s / HBM_PEAK if tr else None,
-                "algorithmic_bytes_per_launch": ab, "avg_launch_ms": avg_s * 1e3,
-                "ms_per_step": ms_tot / max(1, ksteps)}
+        r = {"kernel": k, "bound": "hbm", "achieved": ab / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
+             "unit": "GB/s", "frac": ab / avg_s / HBM_PEAK, "traffic": tr,
+             # the same launch time against the HBM bytes the counters measured
+             "frac_of_measured_traffic": tr / avg_s / HBM_PEAK if tr else None,
+             "algorithmic_bytes_per_launch": ab, "avg_launch_ms": avg_s * 1e3,
+             "ms_per_step": ms_tot / max(1, ksteps)}
+        if k == "lc_chi2_grad":  # its gathers, served by the caches
+            r["tap_bytes_per_launch"] = tap_bytes(4, ncurves)
+            r["tap_rate_GBps"] = tap_bytes(4, ncurves) / avg_s / 1e9
+        return r
 
     # the dominant kernel by time per step, and every kernel with algorithmic bytes
     timed = [k for k in kern if algorithmic_bytes(k, 4, ncurves)]
