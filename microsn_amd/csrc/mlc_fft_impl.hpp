@@ -638,20 +638,21 @@ __global__ __launch_bounds__(CtGeom<N>::G) void k_node_rows(
 // Node-map rows straight from the map's row spectrum: the y-filter of
 // k_node_yfilter and the row transform of k_node_rows in one kernel, so the
 // two node spectra never leave the chip.  A workgroup of R transform groups
-// (R CtGeom<N>::G threads) owns R consecutive rows y0 .. y0+R-1.  First every
-// thread runs the y-filter for its column pairs (u, u+1), u even, u < N/2,
-// and the R rows.  A pair is one 16-byte load in FP32 (rows of G and of the
-// taps have even strides), which halves the vector-memory instructions, the
-// limit of this phase.  The inputs G[y0+i-k] and G[y0+i+k+1] of a block of
-// kNfK taps sit in two register windows that slide by kNfK rows per block;
-// the next block's rows and taps are loaded before the current block's
-// arithmetic.  Each thread puts Z[u] = Hv + i Hd and Z[N-u] into the R row
-// buffers in LDS; then group g transforms row y0+g and writes it into the
-// tiled node map.  Taps tp are the (value, derivative) pairs in the working
-// precision, hq rows (h rounded up to a multiple of kNfK, zeros at k >= h);
-// the sums, their order (k ascending) and the final products are those of
-// k_node_yfilter and k_node_rows.  Workgroups run XCD by XCD over
-// consecutive row blocks, so the halo rows of G are shared in one L2.
+// (R CtGeom<N>::G threads) owns R H consecutive rows y0 .. y0+RH-1.  First
+// every thread runs the y-filter (node_filter) for its column pairs (u,
+// u+1), u even, u < N/2, and all RH rows.  A pair is one 16-byte load in
+// FP32 (rows of G and of the taps have even strides), which halves the
+// vector-memory instructions, and each row and tap load serves RH output
+// rows: the L2-to-L1 traffic of these loads, (2h + RH - 1 + h) / RH rows
+// per output row, is the limit of this phase.  Each thread puts Z[u] = Hv +
+// i Hd and Z[N-u] of the first R rows into the R row buffers in LDS (H = 2:
+// the other R rows' sums wait in registers); then group g transforms row
+// y0+g and the workgroup writes the R rows into the tiled node map; with H
+// = 2 the held rows follow through the same buffers.  Taps tp are the
+// (value, derivative) pairs in the working precision, hq >= h rows (zeros
+// at k >= h); the sums, their order (k ascending) and the final products
+// are those of k_node_yfilter and k_node_rows.  Workgroups run XCD by XCD
+// over consecutive row blocks, so the halo rows of G are shared in one L2.
 constexpr int kNfK = 2;
 
 template <class T>
@@ -659,97 +660,144 @@ struct alignas(2 * sizeof(cpx<T>)) CPair {
   cpx<T> a, b;
 };
 
-template <class T, int N, int R>
+// f(integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, int I = 0, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<N, I + 1>(f);
+  }
+}
+
+// A raw buffer resource over a device array of fewer than 2^32 bytes: a load
+// takes a per-lane byte offset and a wave-uniform byte offset, so a row
+// start is scalar arithmetic and the lane's column one fixed register.
+struct BufView {
+  __amdgpu_buffer_rsrc_t rs;
+};
+__device__ __forceinline__ BufView buf_view(const void* p, uint32_t bytes) {
+  // dword 3 = 0x00020000: the raw 32-bit data format of gfx9 buffer loads
+  return BufView{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000)};
+}
+template <class T>
+__device__ __forceinline__ CPair<T> buf_pair(const BufView& b, uint32_t voff, uint32_t soff) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(CPair<T>, __builtin_amdgcn_raw_buffer_load_b128(b.rs, voff, soff, 0));
+  } else {
+    struct W2 {
+      v4u p, q;
+    } w{__builtin_amdgcn_raw_buffer_load_b128(b.rs, voff, soff, 0),
+        __builtin_amdgcn_raw_buffer_load_b128(b.rs, voff + 16u, soff, 0)};
+    return __builtin_bit_cast(CPair<T>, w);
+  }
+}
+
+// The y-filter of the fused node kernels for one column pair (u, u+1) and
+// the R rows y0 .. y0+R-1:
+//   accv[i][e] = sum_{k < hq} tv_k[u+e] (G[y0+i-k][u+e] + G[y0+i+k+1][u+e])
+// (accd with the derivative taps), k ascending, one product-add per tap.
+// The left rows G[y0+R-1-m], the right rows G[y0+1+m] (m = 0, 1, ...) and the
+// tap rows sit in rings of S = R + P register slots, slot m mod S.  Tap k
+// reads slots k .. k+R-1 of both rings; before its arithmetic it issues the
+// loads of row m = k+R+P-1 (into the slot tap k-1 freed) and of tap k+P, so
+// P taps of loads are in flight.  The loop body is unrolled over S taps, so
+// every slot keeps its register and nothing is moved.  Rows and taps past
+// the last ones read repeat the last ones (never used).  Row starts are
+// wave-uniform byte offsets (rowb bytes per row of G, tapb per tap row);
+// voff is the lane's column byte offset.  Rows wrap around Ny (Ny >= h+16).
+template <class T, int R, int P>
+__device__ __forceinline__ void node_filter(const BufView& gb, const BufView& tb, uint32_t voff, uint32_t rowb,
+                                            uint32_t tapb, int y0, int Ny, int hq, cpx<T> (&accv)[R][2],
+                                            cpx<T> (&accd)[R][2]) {
+  using P2 = CPair<T>;
+  constexpr int S = R + P;
+  const int mmax = hq + R - 2;
+  auto wrap = [&](int r) { return r < 0 ? r + Ny : (r >= Ny ? r - Ny : r); };
+  auto lrow = [&](int m) { return (uint32_t)wrap(y0 + R - 1 - min(m, mmax)) * rowb; };
+  auto rrow = [&](int m) { return (uint32_t)wrap(y0 + 1 + min(m, mmax)) * rowb; };
+  auto trow = [&](int k) { return (uint32_t)min(k, hq - 1) * tapb; };
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) accv[i][e] = accd[i][e] = mk<T>(0, 0);
+  P2 Ls[S], Rs[S], Ts[S];
+#pragma unroll
+  for (int m = 0; m < S - 1; ++m) {
+    Ls[m] = buf_pair<T>(gb, voff, lrow(m));
+    Rs[m] = buf_pair<T>(gb, voff, rrow(m));
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) Ts[k] = buf_pair<T>(tb, voff, trow(k));
+  // tap k = kb + j, slot j: loads for later taps, then this tap's sums
+  auto tap = [&](auto jc, int k) {
+    constexpr int j = decltype(jc)::value;
+    Ls[(j + S - 1) % S] = buf_pair<T>(gb, voff, lrow(k + S - 1));
+    Rs[(j + S - 1) % S] = buf_pair<T>(gb, voff, rrow(k + S - 1));
+    Ts[(j + P) % S] = buf_pair<T>(tb, voff, trow(k + P));
+    const P2 tk = Ts[j];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const P2& l = Ls[(j + R - 1 - i) % S];
+      const P2& r = Rs[(j + i) % S];
+      const cpx<T> s0 = add(l.a, r.a), s1 = add(l.b, r.b);
+      accv[i][0] = mk<T>(accv[i][0].x + tk.a.x * s0.x, accv[i][0].y + tk.a.x * s0.y);
+      accd[i][0] = mk<T>(accd[i][0].x + tk.a.y * s0.x, accd[i][0].y + tk.a.y * s0.y);
+      accv[i][1] = mk<T>(accv[i][1].x + tk.b.x * s1.x, accv[i][1].y + tk.b.x * s1.y);
+      accd[i][1] = mk<T>(accd[i][1].x + tk.b.y * s1.x, accd[i][1].y + tk.b.y * s1.y);
+    }
+  };
+  using std::integral_constant;
+  // whole rounds of S taps in one basic block (no exit between taps, so the
+  // compiler keeps every load where it is issued), then the last hq mod S
+  int kb = 0;
+  for (; kb + S <= hq; kb += S) {
+    static_for<S>([&](auto jc) { tap(jc, kb + decltype(jc)::value); });
+  }
+  static_for<S - 1>([&](auto jc) {
+    if (kb + decltype(jc)::value < hq) tap(jc, kb + decltype(jc)::value);
+  });
+}
+// taps of loads in flight: three in FP32 (60 VGPRs of rings at R = 2), one
+// in FP64 (pairs take twice the registers)
+template <class T, int R, int H>
+constexpr int node_filter_depth() {
+  return sizeof(T) == 4 ? (H > 1 ? 1 : (R >= 4 ? 2 : 3)) : 1;
+}
+
+template <class T, int N, int R, int H>
 __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
-    const cpx<T>* __restrict__ Gs, int ldg, const cpx<T>* __restrict__ tp, int ldt, int h, int hq,
+    const cpx<T>* __restrict__ Gs, int ldg, const cpx<T>* __restrict__ tp, int ldt, int hq,
     const double* __restrict__ phx, cpx<T>* __restrict__ out, int Ny, const double* __restrict__ shift,
     const cpx<T>* __restrict__ tw) {
   using Gm = CtGeom<N>;
-  using P2 = CPair<T>;
-  constexpr int G = Gm::G, NT = G * R, PPT = (N / 4) / NT, NW = R + kNfK - 1;
+  constexpr int G = Gm::G, NT = G * R, PPT = (N / 4) / NT, RF = R * H;
   static_assert((N / 4) % NT == 0, "column pairs split evenly over the threads");
+  static_assert(H == 1 || H == 2, "one or two rounds of R rows");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* lo = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* hi = lo + Gm::NLO;
   cpx<T>* bufs = lo + Gm::NTW;  // R buffers of Gm::LDS elements
-  const int nq = (Ny + R - 1) / R;
+  const int nq = (Ny + RF - 1) / RF;
   const int q = xcd_block(nq);
   if (q >= nq) return;
-  const int y0 = q * R, t = threadIdx.x;
+  const int y0 = q * RF, t = threadIdx.x;
   for (int i = t; i < Gm::NTW; i += NT) lo[i] = tw[i];
   const double sc = 0.5 / (double)N;
-  // the pair at row r of the spectrum, r in [-Ny, 2 Ny); the pair of taps k
-  auto gpair = [&](int r, int u) -> P2 {
-    r = r < 0 ? r + Ny : (r >= Ny ? r - Ny : r);
-    return *reinterpret_cast<const P2*>(Gs + (size_t)r * ldg + u);
-  };
-  auto tpair = [&](int k, int u) -> P2 { return *reinterpret_cast<const P2*>(tp + (size_t)k * ldt + u); };
-#pragma unroll
-  for (int c = 0; c < PPT; ++c) {
+  const uint32_t rowb = (uint32_t)ldg * (uint32_t)sizeof(cpx<T>), tapb = (uint32_t)ldt * (uint32_t)sizeof(cpx<T>);
+  const BufView gb = buf_view(Gs, (uint32_t)Ny * rowb), tb = buf_view(tp, (uint32_t)hq * tapb);
+  // Z[u] = Hv + i Hd and Z[N-u] of rows r0 .. r0+R-1 from the filter sums
+  // of column pair c into the R row buffers
+  auto put = [&](int c, const cpx<T> (&av)[RF][2], const cpx<T> (&ad)[RF][2], int r0) {
     const int u = 2 * (t + c * NT);
-    cpx<T> ph[2];
 #pragma unroll
-    for (int e = 0; e < 2; ++e)
-      ph[e] = mk<T>((T)(phx[2 * (u + e)] * sc), (T)(phx[2 * (u + e) + 1] * sc));
-    cpx<T> accv[R][2], accd[R][2];
+    for (int e = 0; e < 2; ++e) {
+      const int ue = u + e;
+      const cpx<T> ph = mk<T>((T)(phx[2 * ue] * sc), (T)(phx[2 * ue + 1] * sc));
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) accv[i][e] = accd[i][e] = mk<T>(0, 0);
-    // L[j] = G[y0 - kb - (kNfK-1) + j], Rw[j] = G[y0 + kb + 1 + j], j < NW
-    P2 L[NW], Rw[NW], tk[kNfK];
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      L[j] = gpair(y0 - (kNfK - 1) + j, u);
-      Rw[j] = gpair(y0 + 1 + j, u);
-    }
-#pragma unroll
-    for (int kk = 0; kk < kNfK; ++kk) tk[kk] = tpair(kk, u);
-    for (int kb = 0; kb < hq; kb += kNfK) {
-      // the next block's new rows and taps, in flight during this block's sums
-      const bool more = kb + kNfK < hq;
-      P2 nL[kNfK], nR[kNfK], nt[kNfK];
-      if (more) {
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) {
-          nL[j] = gpair(y0 - kb - kNfK - (kNfK - 1) + j, u);
-          nR[j] = gpair(y0 + kb + kNfK + 1 + (R - 1) + j, u);
-          nt[j] = tpair(kb + kNfK + j, u);
-        }
-      }
-#pragma unroll
-      for (int kk = 0; kk < kNfK; ++kk)
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          const P2& l = L[i - kk + kNfK - 1];
-          const P2& r = Rw[i + kk];
-          const cpx<T> s0 = add(l.a, r.a), s1 = add(l.b, r.b);
-          accv[i][0] = mk<T>(accv[i][0].x + tk[kk].a.x * s0.x, accv[i][0].y + tk[kk].a.x * s0.y);
-          accd[i][0] = mk<T>(accd[i][0].x + tk[kk].a.y * s0.x, accd[i][0].y + tk[kk].a.y * s0.y);
-          accv[i][1] = mk<T>(accv[i][1].x + tk[kk].b.x * s1.x, accv[i][1].y + tk[kk].b.x * s1.y);
-          accd[i][1] = mk<T>(accd[i][1].x + tk[kk].b.y * s1.x, accd[i][1].y + tk[kk].b.y * s1.y);
-        }
-      if (more) {
-        // slide: L moves up kNfK rows, Rw down kNfK rows
-#pragma unroll
-        for (int j = NW - 1; j >= kNfK; --j) L[j] = L[j - kNfK];
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) L[j] = nL[j];
-#pragma unroll
-        for (int j = 0; j + kNfK < NW; ++j) Rw[j] = Rw[j + kNfK];
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) Rw[R - 1 + j] = nR[j];
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) tk[j] = nt[j];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      cpx<T>* buf = bufs + i * Gm::LDS;
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int ue = u + e;
-        const cpx<T> a = mul(accv[i][e], ph[e]), b = mul(accd[i][e], ph[e]);
+      for (int i = 0; i < R; ++i) {
+        cpx<T>* buf = bufs + i * Gm::LDS;
+        const cpx<T> a = mul(av[r0 + i][e], ph), b = mul(ad[r0 + i][e], ph);
         if (ue == 0) {
           buf[pad16(0)] = mk<T>(a.x, b.x);
         } else {
@@ -758,28 +806,69 @@ __global__ __launch_bounds__(CtGeom<N>::G * R) void k_node_fused(
         }
       }
     }
-  }
-  if (t < R) bufs[t * Gm::LDS + pad16(N / 2)] = mk<T>(0, 0);  // Nyquist column
-  __syncthreads();
-  const int g = t / G, tl = t % G;
-  ct_fft<T, N, true>(bufs + g * Gm::LDS, lo, hi, tl);  // ends with a barrier: all R rows done
-  // Stores of whole pixel pairs: chunk s is pixels (x, x+1), x = 4 (s / 2R)
-  // + 2 (s % 2), of row y0 + (s % 2R) / 2, one 16-byte store; the 2R chunks
-  // of one tile are consecutive lanes and fill the workgroup's R rows of that
-  // 4 x 4 tile with contiguous bytes (64 bytes for R = 2, the whole 128-byte
-  // tile for R = 4), instead of 32-byte pieces per row.
-  const T sh = shift ? (T)*shift : (T)0;
-  typedef T v4 __attribute__((ext_vector_type(4)));
+  };
+  // transform the R buffers (rows ya .. ya+R-1) and store them: chunk s is
+  // pixels (x, x+1), x = 4 (s / 2R) + 2 (s % 2), of row ya + (s % 2R) / 2,
+  // one 16-byte store; the 2R chunks of one tile are consecutive lanes and
+  // fill the workgroup's R rows of that 4 x 4 tile with contiguous bytes (64
+  // bytes for R = 2, the whole 128-byte tile for R = 4), instead of 32-byte
+  // pieces per row.
+  auto transform_store = [&](int ya) {
+    if (t < R) bufs[t * Gm::LDS + pad16(N / 2)] = mk<T>(0, 0);  // Nyquist column
+    __syncthreads();
+    const int g = t / G, tl = t % G;
+    ct_fft<T, N, true>(bufs + g * Gm::LDS, lo, hi, tl);  // ends with a barrier: all R rows done
+    const T sh = shift ? (T)*shift : (T)0;
+    typedef T v4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-  for (int i = 0; i < N / (2 * G); ++i) {
-    const int sidx = t + i * NT, tile = sidx / (2 * R), part = sidx % (2 * R);
-    const int row = part >> 1, x = 4 * tile + 2 * (part & 1), y = y0 + row;
-    const cpx<T>* b = bufs + row * Gm::LDS;
-    const cpx<T> z0 = b[pad16(x)], z1 = b[pad16(x + 1)];
-    if (y < Ny) {
-      const v4 val = {z0.x + sh, z0.y, z1.x + sh, z1.y};
-      node_store(reinterpret_cast<v4*>(out + tile_offset(x, y, N)), val);
+    for (int i = 0; i < N / (2 * G); ++i) {
+      const int sidx = t + i * NT, tile = sidx / (2 * R), part = sidx % (2 * R);
+      const int row = part >> 1, x = 4 * tile + 2 * (part & 1), y = ya + row;
+      const cpx<T>* b = bufs + row * Gm::LDS;
+      const cpx<T> z0 = b[pad16(x)], z1 = b[pad16(x + 1)];
+      if (y < Ny) {
+        const v4 val = {z0.x + sh, z0.y, z1.x + sh, z1.y};
+        node_store(reinterpret_cast<v4*>(out + tile_offset(x, y, N)), val);
+      }
     }
+  };
+  // H = 2: rows y0+R .. y0+2R-1 are filtered with the first R (each row and
+  // tap load then serves 2R output rows) and wait in registers while the
+  // first R are transformed
+  cpx<T> kv[H > 1 ? PPT : 1][H > 1 ? R : 1][2], kd[H > 1 ? PPT : 1][H > 1 ? R : 1][2];
+#pragma unroll
+  for (int c = 0; c < PPT; ++c) {
+    const int u = 2 * (t + c * NT);
+    cpx<T> accv[RF][2], accd[RF][2];
+    node_filter<T, RF, node_filter_depth<T, RF, H>()>(gb, tb, (uint32_t)u * (uint32_t)sizeof(cpx<T>), rowb, tapb,
+                                                   y0, Ny, hq, accv, accd);
+    put(c, accv, accd, 0);
+    if constexpr (H > 1) {
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          kv[c][i][e] = accv[R + i][e];
+          kd[c][i][e] = accd[R + i][e];
+        }
+    }
+  }
+  transform_store(y0);
+  if constexpr (H > 1) {
+    __syncthreads();  // every store has read the buffers
+#pragma unroll
+    for (int c = 0; c < PPT; ++c) {
+      cpx<T> av[RF][2], ad[RF][2];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          av[R + i][e] = kv[c][i][e];
+          ad[R + i][e] = kd[c][i][e];
+        }
+      put(c, av, ad, R);
+    }
+    transform_store(y0 + R);
   }
 }
 
@@ -799,8 +888,7 @@ __global__ __launch_bounds__(G * R) void k_node_fused_mix(
     const cpx<T>* __restrict__ Gs, int ldg, const cpx<T>* __restrict__ tp, int ldt, int hq,
     const double* __restrict__ phx, cpx<T>* __restrict__ out, int Ny, const double* __restrict__ shift,
     const cpx<T>* __restrict__ twN) {
-  using P2 = CPair<T>;
-  constexpr int NT = G * R, QL = N / 4, LDSN = N + N / 16, NW = R + kNfK - 1;
+  constexpr int NT = G * R, QL = N / 4, LDSN = N + N / 16;
   static_assert(N % 4 == 0 && G % 64 == 0 && 16 * G >= N, "row geometry");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* qt = reinterpret_cast<cpx<T>*>(smem);  // W_N^m, m < N/4
@@ -811,11 +899,8 @@ __global__ __launch_bounds__(G * R) void k_node_fused_mix(
   const int y0 = q * R, t = threadIdx.x;
   for (int i = t; i < QL; i += NT) qt[i] = twN[i];
   const double sc = 0.5 / (double)N;
-  auto gpair = [&](int r, int u) -> P2 {
-    r = r < 0 ? r + Ny : (r >= Ny ? r - Ny : r);
-    return *reinterpret_cast<const P2*>(Gs + (size_t)r * ldg + u);
-  };
-  auto tpair = [&](int k, int u) -> P2 { return *reinterpret_cast<const P2*>(tp + (size_t)k * ldt + u); };
+  const uint32_t rowb = (uint32_t)ldg * (uint32_t)sizeof(cpx<T>), tapb = (uint32_t)ldt * (uint32_t)sizeof(cpx<T>);
+  const BufView gb = buf_view(Gs, (uint32_t)Ny * rowb), tb = buf_view(tp, (uint32_t)hq * tapb);
   for (int cp = t; cp < N / 4; cp += NT) {
     const int u = 2 * cp;
     cpx<T> ph[2];
@@ -823,54 +908,8 @@ __global__ __launch_bounds__(G * R) void k_node_fused_mix(
     for (int e = 0; e < 2; ++e)
       ph[e] = mk<T>((T)(phx[2 * (u + e)] * sc), (T)(phx[2 * (u + e) + 1] * sc));
     cpx<T> accv[R][2], accd[R][2];
-#pragma unroll
-    for (int i = 0; i < R; ++i)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) accv[i][e] = accd[i][e] = mk<T>(0, 0);
-    P2 L[NW], Rw[NW], tk[kNfK];
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      L[j] = gpair(y0 - (kNfK - 1) + j, u);
-      Rw[j] = gpair(y0 + 1 + j, u);
-    }
-#pragma unroll
-    for (int kk = 0; kk < kNfK; ++kk) tk[kk] = tpair(kk, u);
-    for (int kb = 0; kb < hq; kb += kNfK) {
-      const bool more = kb + kNfK < hq;
-      P2 nL[kNfK], nR[kNfK], nt[kNfK];
-      if (more) {
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) {
-          nL[j] = gpair(y0 - kb - kNfK - (kNfK - 1) + j, u);
-          nR[j] = gpair(y0 + kb + kNfK + 1 + (R - 1) + j, u);
-          nt[j] = tpair(kb + kNfK + j, u);
-        }
-      }
-#pragma unroll
-      for (int kk = 0; kk < kNfK; ++kk)
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          const P2& l = L[i - kk + kNfK - 1];
-          const P2& r = Rw[i + kk];
-          const cpx<T> s0 = add(l.a, r.a), s1 = add(l.b, r.b);
-          accv[i][0] = mk<T>(accv[i][0].x + tk[kk].a.x * s0.x, accv[i][0].y + tk[kk].a.x * s0.y);
-          accd[i][0] = mk<T>(accd[i][0].x + tk[kk].a.y * s0.x, accd[i][0].y + tk[kk].a.y * s0.y);
-          accv[i][1] = mk<T>(accv[i][1].x + tk[kk].b.x * s1.x, accv[i][1].y + tk[kk].b.x * s1.y);
-          accd[i][1] = mk<T>(accd[i][1].x + tk[kk].b.y * s1.x, accd[i][1].y + tk[kk].b.y * s1.y);
-        }
-      if (more) {
-#pragma unroll
-        for (int j = NW - 1; j >= kNfK; --j) L[j] = L[j - kNfK];
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) L[j] = nL[j];
-#pragma unroll
-        for (int j = 0; j + kNfK < NW; ++j) Rw[j] = Rw[j + kNfK];
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) Rw[R - 1 + j] = nR[j];
-#pragma unroll
-        for (int j = 0; j < kNfK; ++j) tk[j] = nt[j];
-      }
-    }
+    node_filter<T, R, node_filter_depth<T, R, 1>()>(gb, tb, (uint32_t)u * (uint32_t)sizeof(cpx<T>), rowb, tapb,
+                                                y0, Ny, hq, accv, accd);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       cpx<T>* buf = bufs + i * LDSN;
@@ -1454,37 +1493,43 @@ bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx,
 // does not fit in LDS, fewer than h + 16 rows, or taps that the y-filter
 // would not stage either (so both paths take the column FFTs for the same
 // profiles).  Rows per workgroup: 4 up to 1024 columns, else 2 where two
-// row buffers fit, else 1.  At 4096 FP32 columns two rows (70 KB of LDS, two
-// workgroups per CU, one filtering while the other transforms) beat four
-// rows (140 KB, one workgroup per CU, fewer halo and tap reads but no
-// overlap).  ldg and ldt (row strides of Gs and tp) must be even.
+// row buffers fit, else 1.  At 4096 FP32 columns two row buffers (70 KB of
+// LDS, two workgroups per CU, one filtering while the other transforms)
+// beat four (140 KB, one workgroup per CU, no overlap); at 2048 and 4096
+// FP32 columns each workgroup filters four rows through its two buffers
+// (H = 2), so each row and tap load serves four output rows.  ldg and ldt
+// (row strides of Gs and tp) must be even, and the row spectrum below 4 GiB.
 template <class T>
 bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int ldt, int h, T* out, int Nx,
                 int Ny, const double* shift) {
-  const int hp = (h + kYfK - 1) / kYfK * kYfK, hq = (h + kNfK - 1) / kNfK * kNfK;
+  const int hp = (h + kYfK - 1) / kYfK * kYfK;
   if (sizeof(cpx<T>) * (size_t)(kYfRows + 3 * hp) * kYfCols > kLdsBytes) return false;
   if (Ny < h + 16) return false;
+  // the filter's buffer loads address G with 32-bit byte offsets
+  if ((size_t)Ny * ldg * sizeof(cpx<T>) >= (size_t(1) << 32)) return false;
   bool done = false;
   auto go = [&](auto nc) {
     constexpr int N = decltype(nc)::value;
     using Gm = CtGeom<N>;
     auto lds_for = [](int r) { return sizeof(cpx<T>) * (size_t)(Gm::NTW + r * Gm::LDS); };
-    auto run = [&](auto rc) {
-      constexpr int R = decltype(rc)::value;
+    auto run = [&](auto rc, auto hc) {
+      constexpr int R = decltype(rc)::value, H = decltype(hc)::value;
       const size_t lds = lds_for(R);
-      auto k = k_node_fused<T, N, R>;
+      auto k = k_node_fused<T, N, R, H>;
       allow_lds(k, lds);
-      const int nq = (Ny + R - 1) / R;
+      const int nq = (Ny + R * H - 1) / (R * H);
       MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(Gm::G * R), lds, Gs, ldg, tp, ldt,
-                 h, hq, cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
+                 h, cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
       done = true;
     };
     using std::integral_constant;
     if constexpr (N <= 1024) {
-      run(integral_constant<int, 4>{});
+      run(integral_constant<int, 4>{}, integral_constant<int, 1>{});
+    } else if constexpr (sizeof(T) == 4 && (N == 2048 || N == 4096)) {
+      run(integral_constant<int, 2>{}, integral_constant<int, 2>{});
     } else {
-      if (lds_for(2) <= kLdsBytes) run(integral_constant<int, 2>{});
-      else if (lds_for(1) <= kLdsBytes) run(integral_constant<int, 1>{});
+      if (lds_for(2) <= kLdsBytes) run(integral_constant<int, 2>{}, integral_constant<int, 1>{});
+      else if (lds_for(1) <= kLdsBytes) run(integral_constant<int, 1>{}, integral_constant<int, 1>{});
     }
   };
   // widths that are not powers of two: G threads per row, R rows per
@@ -1497,7 +1542,7 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int l
     auto k = k_node_fused_mix<T, N, R, G>;
     allow_lds(k, lds);
     const int nq = (Ny + R - 1) / R;
-    MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(G * R), lds, Gs, ldg, tp, ldt, hq,
+    MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(G * R), lds, Gs, ldg, tp, ldt, h,
                cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, twiddles<T>(ctx, N));
     done = true;
   };
