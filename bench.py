@@ -685,14 +685,18 @@ def cpu_baseline(m, params, prof, times, mobs, sig, bins):
                        f"{t_step:.1f} s in all")}, chi2
 
 
-def pmc_step():
+def pmc_step(lanes, grid_multiple):
     """Two FP32 steps of the workload, then a 512 MiB device-to-device copy
-    whose FETCH_SIZE / WRITE_SIZE calibrate the byte counters."""
+    whose FETCH_SIZE / WRITE_SIZE calibrate the byte counters.  The band
+    launches use the given shape (the one the bench's own tuner keeps for this
+    workload, reported as band_launch_shape): under counter collection the
+    tuner's timings, and so its choice, would differ from run to run."""
     import torch
 
     import microsn_amd.api as api
 
     ctx = api.Context(0)
+    ctx.tune_force(lanes, grid_multiple)
     m_dev = torch.tensor(make_map(), device="cuda")
     params, prof, times, _, _ = make_inputs()
     mobs = [np.full(len(t), -1.7) for t in times]
@@ -724,6 +728,10 @@ def main():
     ap.add_argument("--pmc-step", action="store_true",
                     help="run two FP32 steps and a 512 MiB device copy (the byte-counter "
                          "calibration) for rocprofv3 --pmc, print nothing else")
+    ap.add_argument("--pmc-lanes", type=int, default=16,
+                    help="band launch shape of --pmc-step: lanes per curve")
+    ap.add_argument("--pmc-grid", type=int, default=1,
+                    help="band launch shape of --pmc-step: grid as a multiple of one fill")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -736,7 +744,7 @@ def main():
     import microsn_amd.lightcurves as lcm
 
     if args.pmc_step:
-        pmc_step()
+        pmc_step(args.pmc_lanes, args.pmc_grid)
         return
     if args.extras_only:
         ctx = api.Context(0)
@@ -786,6 +794,8 @@ def main():
     ksteps = max(1, min(args.steps, 5))
     dt32_ev, _, kern = time_steps(ctx, ws32, ksteps, 0, plain=False, timing=True)
     chi32, grad32, lo32, npb32, be32 = out32
+    band_shape = dict(zip(("lanes", "grid_multiple"),
+                          ctx.tune_lookup(ctx.TUNE_KIND_BANDS, "f32", EPOCHS, float(N) * N)))
 
     # ---- FP64 plain path (same workload)
     plain = None
@@ -876,6 +886,7 @@ def main():
         "kernels_ms_per_step": {k: v[0] / ksteps for k, v in sorted(kern.items())},
         "ms_per_step_with_kernel_events": dt32_ev * 1e3,
         "hip_graph": graph_info,
+        "band_launch_shape": band_shape,
         "setup_s": t_setup,
         "extras": extras,
     }
