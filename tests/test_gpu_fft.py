@@ -180,6 +180,34 @@ def test_node_only_request_fused_kernel(ctx, orc, Ny, Nx, shape, rhalf, prec):
     assert np.max(np.abs(nd[..., 1] - dref)) < tol * 20 * max(np.max(np.abs(ref)), 1.0)
 
 
+@pytest.mark.parametrize("Nx,Ny", [(2048, 48), (2560, 40), (4096, 32)])
+@pytest.mark.parametrize("shape,rhalf", [("uniform", 1.0),      # h = 5: whole rounds of taps only
+                                         ("uniform", 1.5),      # h = 6: one tap after a round
+                                         ("gaussian", 1.0),     # h = 2: a partial round only
+                                         ("exponential", 1.5),  # h = 3
+                                         ("uniform", 4.0),      # h = 9: a round and four taps
+                                         ("gaussian", 6.0)])    # h = 13: two rounds and three taps
+def test_node_filter_rounds_and_tails(ctx, orc, Nx, Ny, shape, rhalf):
+    """The fused node kernels filter with register rings of five tap slots
+    (four rows per workgroup at 2048 and 4096 FP32 columns, two at 2560):
+    tap counts that leave every remainder modulo five, against the separate
+    y-filter and row kernels (within 4 ulp of the map's scale) and the
+    oracle."""
+    rng = np.random.default_rng(Nx + int(10 * rhalf))
+    m = _smooth_map(rng, Ny, Nx)
+    spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), "f32")
+    alone = spec.convolve(shape, rhalf, 1.0, value=False, node=True)
+    _, both = spec.convolve(shape, rhalf, 1.0, value=True, node=True)
+    assert float((alone - both).abs().max()) <= 4 * 2.0 ** -23 * float(both.abs().max())
+    prof, dprof = orc.profile(shape, rhalf, 1.0, deriv=True)
+    ref = orc.fft_convolve(m, orc.set_kernel(prof, Nx, Ny))
+    dref = orc.fft_convolve(m, orc.set_kernel(dprof, Nx, Ny))
+    nd = alone.cpu().numpy().astype(np.float64)
+    assert _rel(nd[..., 0], ref) < 2e-6
+    assert np.max(np.abs(nd[..., 1] - dref)) < 2e-6 * 20 * max(np.max(np.abs(ref)), 1.0)
+    spec.close()
+
+
 @pytest.mark.parametrize("Ny,Nx", [(128, 160), (128, 256)])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_convolve_nodes_equals_one_by_one(ctx, Ny, Nx, prec):
@@ -223,7 +251,7 @@ def test_context_close_closes_its_maps():
     del spec
 
 
-@pytest.mark.parametrize("Ny,Nx", [(40, 1000), (32, 1280), (24, 3072), (20, 10000)])
+@pytest.mark.parametrize("Ny,Nx", [(40, 1000), (32, 1280), (24, 2560), (24, 3072), (20, 10000)])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_node_map_rows_at_fixed_widths(ctx, orc, Ny, Nx, prec):
     """Node maps (mu, dmu/drhalf) at widths whose rows run the compile-time
