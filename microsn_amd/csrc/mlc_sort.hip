@@ -140,7 +140,14 @@ __global__ __launch_bounds__(kThreads) void k_scan_tiles(unsigned* __restrict__ 
 }
 
 // Stable scatter of one pass.  payload_in == nullptr means the identity.  A
-// trivial pass copies keys and payload unchanged.
+// trivial pass copies keys and payload unchanged.  The keys of a tile are
+// ranked as k_seg_sort ranks a segment's: every wave counts the digits of
+// all R rounds at once (one barrier for the whole tile), thread t turns
+// digit t's counts into an exclusive prefix over (round, wave), which is
+// tile order, and every key gets its slot in the tile's digit order before
+// any is staged.  The counts are 16-bit (a tile holds 256 R <= 4096 keys)
+// and share LDS with the staged keys, which are written only after every
+// slot is known: 50 KB of LDS, three workgroups per CU.
 template <int R>
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict__ kin,
                                                       const unsigned* __restrict__ pin,
@@ -150,6 +157,8 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
                                                       const unsigned* __restrict__ offsets,
                                                       const unsigned* __restrict__ hist_pass) {
   constexpr int kRounds = R, kTile = kThreads * R;
+  static_assert(kTile <= 65535, "16-bit tile ranks");
+  static_assert(kThreads == 256, "one thread per digit");
   if (pass_trivial(hist_pass, n)) {
     const int64_t base = (int64_t)blockIdx.x * kTile;
 #pragma unroll
@@ -162,93 +171,94 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
     }
     return;
   }
-  __shared__ uint64_t sk[kTile];
+  // cnt[r][w][d] (ranking) and the staged keys (placement) share this space
+  __shared__ __align__(16) uint64_t sk[kTile];
+  static_assert(sizeof(unsigned short) * kRounds * kWaves * 256 <= sizeof(uint64_t) * kTile,
+                "the rank counts fit where the keys are staged");
+  unsigned short(*cnt)[kWaves][256] = reinterpret_cast<unsigned short(*)[kWaves][256]>(sk);
   __shared__ unsigned sp[kTile];
-  __shared__ unsigned wave_cnt[kWaves][256];
-  __shared__ unsigned run[256];    // keys of each digit ranked so far in this tile
+  __shared__ unsigned wsum[kWaves];
   __shared__ unsigned start[256];  // first LDS slot of each digit (exclusive scan)
+  __shared__ unsigned goff[256];   // where this tile's run of each digit starts in kout
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t base = (int64_t)blockIdx.x * kTile;
-  run[t] = 0;
-  for (int i = t; i < kWaves * 256; i += kThreads) (&wave_cnt[0][0])[i] = 0;
-
+  for (int i = t; i < kRounds * kWaves * 256 / 8; i += kThreads)
+    reinterpret_cast<uint4*>(sk)[i] = make_uint4(0u, 0u, 0u, 0u);
+  goff[t] = offsets[(size_t)t * ntiles + blockIdx.x];
   uint64_t key[kRounds];
-  unsigned pay[kRounds], rank[kRounds], dig[kRounds];
+  unsigned pay[kRounds], dig[kRounds], slot[kRounds];
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
-    int64_t i = base + r * kThreads + t;
-    bool ok = i < n;
+    const int64_t i = base + r * kThreads + t;
+    const bool ok = i < n;
     key[r] = ok ? kin[i] : ~0ull;
     pay[r] = ok ? (pin ? pin[i] : (unsigned)i) : 0u;
-    dig[r] = ok ? digit_of(key[r], pass) : 256u;  // 256: padding, ranked last
+    dig[r] = ok ? digit_of(key[r], pass) : 256u;  // 256: padding, not ranked
   }
   __syncthreads();
+  // rank within the wave: lanes of this wave holding the same digit
+  // (padding lanes match nobody); the first of them records the count
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
     const unsigned d = dig[r];
-    // lanes of this wave holding the same digit (padding lanes match nobody)
     uint64_t peers = __ballot(d < 256u);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      uint64_t m = __ballot((d >> b) & 1u);
+      const uint64_t m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
     if (d >= 256u) peers = 0;
-    const unsigned before = (unsigned)__popcll(peers & lt);
-    if (d < 256u && before == 0) wave_cnt[w][d] = (unsigned)__popcll(peers);
-    __syncthreads();
-    if (d < 256u) {
-      unsigned p = run[d] + before;
-      for (int ww = 0; ww < w; ++ww) p += wave_cnt[ww][d];
-      rank[r] = p;
-    }
-    __syncthreads();
-    {
-      unsigned add = 0;
-      for (int ww = 0; ww < kWaves; ++ww) {
-        add += wave_cnt[ww][t];
-        wave_cnt[ww][t] = 0;
-      }
-      run[t] += add;
-    }
-    __syncthreads();
+    slot[r] = (unsigned)__popcll(peers & lt);
+    if (d < 256u && slot[r] == 0) cnt[r][w][d] = (unsigned short)__popcll(peers);
   }
-  // exclusive scan of the tile's digit counts (run[]) -> start[]
-  {
-    unsigned v = run[t];
-    start[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      unsigned a = t >= off ? start[t - off] : 0u;
-      __syncthreads();
-      start[t] += a;
-      __syncthreads();
+  __syncthreads();
+  // thread t owns digit t: exclusive prefix over (round, wave), then the
+  // digit's start from a scan of the totals over the digits
+  unsigned tot = 0;
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r)
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      const unsigned v = cnt[r][ww][t];
+      cnt[r][ww][t] = (unsigned short)tot;
+      tot += v;
     }
-    unsigned incl = start[t];
-    __syncthreads();
-    start[t] = incl - v;
-    __syncthreads();
+  unsigned incl = tot;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
   }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  unsigned pre = 0;
+  for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+  start[t] = pre + incl - tot;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r)
+    if (dig[r] < 256u) slot[r] += start[dig[r]] + cnt[r][w][dig[r]];
+  __syncthreads();  // every slot read its counts: the space now takes the keys
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
     if (dig[r] < 256u) {
-      unsigned slot = start[dig[r]] + rank[r];
-      sk[slot] = key[r];
-      sp[slot] = pay[r];
+      sk[slot[r]] = key[r];
+      sp[slot[r]] = pay[r];
     }
   }
   __syncthreads();
+  // out in runs that are contiguous in memory
   const int valid = (int)min<int64_t>(kTile, n - base);
 #pragma unroll
   for (int r = 0; r < kRounds; ++r) {
-    int slot = r * kThreads + t;
-    if (slot < valid) {
-      uint64_t k = sk[slot];
-      unsigned d = digit_of(k, pass);
-      size_t dst = (size_t)offsets[(size_t)d * ntiles + blockIdx.x] + (slot - start[d]);
+    const int q = r * kThreads + t;
+    if (q < valid) {
+      const uint64_t k = sk[q];
+      const unsigned d = digit_of(k, pass);
+      const size_t dst = (size_t)goff[d] + (unsigned)(q - start[d]);
       kout[dst] = k;
-      pout[dst] = sp[slot];
+      pout[dst] = sp[q];
     }
   }
 }
@@ -473,13 +483,27 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(const uint64_t* __restr
   }
 }
 
-__global__ void k_rank_and_gather(const unsigned* __restrict__ perm, const double* __restrict__ keys,
-                                  int64_t n, unsigned* __restrict__ rank, double* __restrict__ sorted) {
+// The double whose ordered key is k, for every key that determines it: all
+// but the key of +-0.0 (both are 0x8000000000000000) and that of NaN (~0).
+__device__ __forceinline__ double double_of(uint64_t k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+
+// rank[perm[i]] = i and sorted[i] = keys[perm[i]].  The sorted values come
+// from the sorted ordered keys skeys (read in order); only a key that does
+// not determine its double (a zero, whose sign it dropped, or a NaN, whose
+// payload it dropped) is gathered from the original keys.
+__global__ void k_rank_and_gather(const unsigned* __restrict__ perm, const uint64_t* __restrict__ skeys,
+                                  const double* __restrict__ keys, int64_t n, unsigned* __restrict__ rank,
+                                  double* __restrict__ sorted) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    unsigned p = perm[i];
+    const unsigned p = perm[i];
     if (rank) rank[p] = (unsigned)i;
-    if (sorted) sorted[i] = keys[p];
+    if (sorted) {
+      const uint64_t k = skeys[i];
+      sorted[i] = (k == ~0ull || k == 0x8000000000000000ull) ? keys[p] : double_of(k);
+    }
   }
 }
 
@@ -562,8 +586,9 @@ __global__ __launch_bounds__(64) void k_bin_table_waves(const double* __restrict
 }
 
 template <int R>
-void sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* pbuf, unsigned* perm,
-                 unsigned* cnt, const unsigned* hist, int64_t n) {
+// Returns the buffer that holds the sorted keys.
+const uint64_t* sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* pbuf, unsigned* perm,
+                            unsigned* cnt, const unsigned* hist, int64_t n) {
   constexpr int kTile = kThreads * R;
   const int ntiles = (int)((n + kTile - 1) / kTile);
   // eight passes (trivial ones copy): ping-pong so that the last one writes perm
@@ -585,6 +610,7 @@ void sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* pbuf, unsig
     kin = nk;
     pin = pout;
   }
+  return kin;
 }
 
 }  // namespace
@@ -608,6 +634,7 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   ws.commit();
   uint64_t *k0 = ws.get<uint64_t>(ik0), *k1 = ws.get<uint64_t>(ik1);
   unsigned *pbuf = ws.get<unsigned>(ip1), *hist = ws.get<unsigned>(ih), *cnt = ws.get<unsigned>(ic);
+  const uint64_t* skeys = nullptr;  // the sorted ordered keys
   if (small) {
     // segments, then merge rounds; the segment sort writes the payload where
     // an even number of rounds leaves it in perm
@@ -623,18 +650,19 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
       std::swap(kin, kout);
       std::swap(pin, pout);
     }
+    skeys = kin;
   } else {
     // a fill kernel, not a memset node, so that a captured graph replays it
     MLC_LAUNCH(ctx, "sort_zero_hist", k_zero_u32, dim3(8), dim3(256), 0, hist, 8 * 256);
     const int g = (int)std::min<int64_t>(cdiv(n, kThreads * 4), fill_grid(ctx, 1));
     MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
                hist);
-    sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
+    skeys = sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
   }
   if (rank || sorted)
     MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
-               dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, keys,
-               n, rank, sorted);
+               dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, skeys,
+               keys, n, rank, sorted);
 }
 
 // The bin table on the device: d_lower / d_npb from the upper bounds, no
