@@ -345,6 +345,88 @@ __global__ __launch_bounds__(G) void k_row_c2r(const cpx<T>* __restrict__ in0,
   }
 }
 
+// The 10000-wide node rows (MODE 2 of k_row_c2r, ML = 5000) as a
+// persistent kernel, one workgroup per CU: each workgroup transforms rows y,
+// y + s, ... of its XCD's contiguous range (s = workgroups on that XCD) and
+// loads the next row's spectra into registers while it transforms the
+// current one (the FFT reads its twiddles from an LDS copy, so no wait on
+// global loads drains them early).  One row per workgroup, two workgroups per
+// CU with the twiddles read from the global table, loaded, transformed and
+// stored in lockstep, each phase leaving the other units idle.  Same
+// arithmetic as k_row_c2r MODE 2 (c2r_pre, row_fft, the 2-pair stores), so
+// the same bits.
+template <class T, int G, int ML>
+__global__ __launch_bounds__(G) void k_row_c2r_node_pf(const cpx<T>* __restrict__ in0,
+                                                       const cpx<T>* __restrict__ in1, T* __restrict__ out,
+                                                       int Ny, const double* __restrict__ shift, Plan plan,
+                                                       const cpx<T>* __restrict__ twM,
+                                                       const cpx<T>* __restrict__ twN) {
+  constexpr int M = ML, Nx = 2 * ML, W = M + 1, KPT = (M / 2 + G) / G, QL = ML % 4 == 0 ? ML / 4 : ML;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* ql = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* buf0 = ql + QL;
+  cpx<T>* buf1 = buf0 + M;
+  const int g = threadIdx.x, x = blockIdx.x & 7, nwx = gridDim.x >> 3;
+  const int per = (Ny + 7) >> 3, ya = x * per, yb = min(Ny, ya + per);
+  int y = ya + (int)(blockIdx.x >> 3);
+  if (y >= yb) return;  // uniform per workgroup
+  cpx<T> tw[KPT], a0[KPT], b0[KPT], a1[KPT], b1[KPT];  // twN[k]; X0[k], X0[M-k], X1[k], X1[M-k]
+  auto load = [&](int yy) {
+    const cpx<T>* r0 = in0 + (size_t)yy * W;
+    const cpx<T>* r1 = in1 + (size_t)yy * W;
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const int k = g + j * G;
+      if (k <= M / 2) {
+        a0[j] = r0[k];
+        b0[j] = r0[M - k];
+        a1[j] = r1[k];
+        b1[j] = r1[M - k];
+      }
+    }
+  };
+  load(y);
+  for (int i = g; i < QL; i += G) ql[i] = twM[i];
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    const int k = g + j * G;
+    tw[j] = k <= M / 2 ? twN[k] : mk<T>(0, 0);
+  }
+  const T sh = shift ? (T)*shift : (T)0;
+  const NoPad P;
+  const QuarterTw<T> qtw{ql, QL};
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  cpx<T>* o = reinterpret_cast<cpx<T>*>(out);
+  while (true) {
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const int k = g + j * G;
+      if (k <= M / 2) {
+        cpx<T> Zk, Zm;
+        c2r_pre(a0[j], b0[j], tw[j], &Zk, &Zm);
+        buf0[P(k)] = Zk;
+        if (k > 0 && M - k != k) buf0[P(M - k)] = Zm;
+        c2r_pre(a1[j], b1[j], tw[j], &Zk, &Zm);
+        buf1[P(k)] = Zk;
+        if (k > 0 && M - k != k) buf1[P(M - k)] = Zm;
+      }
+    }
+    __syncthreads();
+    const int yn = y + nwx;
+    if (yn < yb) load(yn);  // in flight during the transforms and stores
+    row_fft<T, ML, G, true>(buf0, plan, qtw, P, g);
+    row_fft<T, ML, G, true>(buf1, plan, qtw, P, g);
+    for (int n = g; n < M; n += G) {
+      const cpx<T> a = buf0[P(n)], b = buf1[P(n)];
+      const v4 val = {a.x + sh, b.x, a.y + sh, b.y};
+      *reinterpret_cast<v4*>(o + tile_offset(2 * n, y, Nx)) = val;
+    }
+    if (yn >= yb) break;
+    __syncthreads();  // every store has read the buffers
+    y = yn;
+  }
+}
+
 // ------------------------------------------------------------------ column kernels
 // LDS of a column kernel: [twiddles W_Ny^m, m < tw_len(Ny)] [CB column buffers
 // of LS elements].  FAST == false (columns too long for that layout): the
@@ -1289,11 +1371,13 @@ void row_c2r(mlc_ctx* ctx, const cpx<T>* in0, const cpx<T>* in1, T* out, int Nx,
     if (M == 5000) {
       // 10000-wide node rows: the twiddles from the global table, so that
       // two workgroups (2 x 80 KB of row buffers) share a CU
-      const size_t lds2 = sizeof(cpx<T>) * 2 * (size_t)M;
-      auto k = k_row_c2r<T, 1024, 2, 5000, true>;
+      const size_t lds2 = sizeof(cpx<T>) * ((size_t)tw_len(M) + 2 * (size_t)M);
+      auto k = k_row_c2r_node_pf<T, 1024, 5000>;
       allow_lds(k, lds2);
-      MLC_LAUNCH(ctx, "fft_row_c2r_node", k, dim3(((Ny + 7) / 8) * 8), dim3(1024), lds2, in0, in1, out, Nx, Ny,
-                 shift, p, twM, twN);
+      // one workgroup per CU, a multiple of 8 (one share per XCD)
+      const int grid = (int)std::min<int64_t>(((int64_t)Ny + 7) / 8 * 8, cdiv(fill_grid(ctx, 1), 8) * 8);
+      MLC_LAUNCH(ctx, "fft_row_c2r_node", k, dim3(grid), dim3(1024), lds2, in0, in1, out, Ny, shift, p, twM,
+                 twN);
       return;
     }
   }
