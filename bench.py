@@ -633,15 +633,26 @@ def time_config2(ctx, steps):
 
 
 def cpu_model():
-    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo),
+    with its family, model and stepping numbers (a virtual machine may report
+    only a short name)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if info:
+                        break  # the first processor's block is enough
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                info.setdefault(k, v)
     except OSError:
-        pass
-    return None
+        return None
+    name = info.get("model name")
+    if name is None:
+        return None
+    ids = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 def cpu_baseline(m, params, prof, times, mobs, sig, bins):
