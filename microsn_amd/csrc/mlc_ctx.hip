@@ -109,6 +109,43 @@ __global__ void k_sample_points(const T* __restrict__ conv, int Nx, int Ny, int 
 }
 
 namespace mlc {
+void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::function<void()>& main_work) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(ctx->stream, &st), "hipStreamIsCapturing");
+  if (!ctx->side && st == hipStreamCaptureStatusNone) {
+    // default priority, like every other stream of the library
+    hip_check(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreateWithFlags(side)");
+    hip_check(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)");
+    hip_check(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming), "hipEventCreate(join)");
+  }
+  if (!ctx->side) {
+    beside();
+    main_work();
+    return;
+  }
+  const hipStream_t main = ctx->stream;
+  hip_check(hipEventRecord(ctx->fork_ev, main), "hipEventRecord(fork)");
+  hip_check(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0), "hipStreamWaitEvent(fork)");
+  ctx->stream = ctx->side;
+  try {
+    beside();
+  } catch (...) {
+    ctx->stream = main;
+    (void)hipEventRecord(ctx->join_ev, ctx->side);
+    (void)hipStreamWaitEvent(main, ctx->join_ev, 0);
+    throw;
+  }
+  ctx->stream = main;
+  hip_check(hipEventRecord(ctx->join_ev, ctx->side), "hipEventRecord(join)");
+  try {
+    main_work();
+  } catch (...) {
+    (void)hipStreamWaitEvent(main, ctx->join_ev, 0);
+    throw;
+  }
+  hip_check(hipStreamWaitEvent(main, ctx->join_ev, 0), "hipStreamWaitEvent(join)");
+}
+
 void sample_points(mlc_ctx* ctx, int precision, const void* conv, int Nx, int Ny, int k,
                    int Nprof, int Nloc, const int* lx, const int* ly, double* LC, double* DLC) {
   if (Nloc <= 0) return;
@@ -162,6 +199,12 @@ void mlc_ctx_destroy(mlc_ctx* ctx) {
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
   for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete ctx;

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -89,9 +90,24 @@ struct mlc_ctx {
     size_t cap = 0;
   };
   std::map<int, ConstTable> tables;
+
+  // a second stream (default priority) for work that runs beside the main
+  // stream inside one call, forked and joined by the two events (created on
+  // first use, outside graph capture)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 namespace mlc {
+
+// Runs `beside` on the context's side stream while `main_work` runs on the
+// main stream: the side stream waits for the main stream's work so far, and
+// the main stream waits for `beside` before anything issued after this call.
+// Under graph capture the two become parallel branches of the graph.  The
+// side stream is created on the first call that is not being captured; until
+// then (and without it) the two run one after the other.  Kernels launched by
+// `beside` go to the side stream through ctx->stream.
+void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::function<void()>& main_work);
 
 // Makes `ctx`'s device current for the lifetime of the guard.
 struct DeviceGuard {

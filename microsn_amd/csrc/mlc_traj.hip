@@ -1228,9 +1228,18 @@ LaunchShape tune_pick(mlc_ctx* ctx, uint64_t key, const std::function<void(Launc
 // With `fixed` the call's scratch comes from that buffer instead of the
 // context's workspace (no allocation; too small a buffer is an argument error
 // raised before any launch).
-void run_fused(mlc_ctx* ctx, const Args& a, Workspace* fixed = nullptr) {
+//
+// With `concurrent`, that work (the node maps of the combined call) runs on
+// the context's stream while the curve partition, which needs none of it,
+// runs beside it on the side stream (run_beside); the band launches start
+// when both are done.
+void run_fused(mlc_ctx* ctx, const Args& a, Workspace* fixed = nullptr,
+               const std::function<void()>* concurrent = nullptr) {
   validate_fused(a);
-  if (a.ncurves == 0) return;
+  if (a.ncurves == 0) {
+    if (concurrent) (*concurrent)();
+    return;
+  }
   const int P = a.P, B = a.B, n = a.ncurves;
   PartGeom pg{P, 1, 0, 0.0, 0.0, a.prof, a.params, a.obs, a.nobs, B, a.nepoch, a.obs_off, a.t};
   while (pg.L < 5 && (int64_t)P * pg.C * 4 + 1 <= kMaxBuckets) {
@@ -1261,16 +1270,20 @@ void run_fused(mlc_ctx* ctx, const Args& a, Workspace* fixed = nullptr) {
   double2* rng = ws.get<double2>(irng);
   int* pos_of = ws.get<int>(ipos);
   const size_t hl = sizeof(int) * nq;
-  MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, kColScanQ)), dim3(kColScanQ * kColScanSeg),
-             0, bh, nq, nblk, tot);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_base, dim3(1), dim3(1024), 0, tot, nq, P, pg.C, n, qbase, start,
-             cnt);
-  MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh, qbase, rec,
-             pos_of);
-  if (a.nepoch > 0)
-    MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
-               a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
+  auto partition = [&] {
+    MLC_LAUNCH(ctx, "lc_partition", k_part_count, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh);
+    MLC_LAUNCH(ctx, "lc_partition", k_part_colscan, dim3(cdiv(nq, kColScanQ)), dim3(kColScanQ * kColScanSeg),
+               0, bh, nq, nblk, tot);
+    MLC_LAUNCH(ctx, "lc_partition", k_part_base, dim3(1), dim3(1024), 0, tot, nq, P, pg.C, n, qbase, start,
+               cnt);
+    MLC_LAUNCH(ctx, "lc_partition", k_part_fill, dim3(nblk), dim3(kPartThreads), hl, pg, n, bh, qbase, rec,
+               pos_of);
+    if (a.nepoch > 0)
+      MLC_LAUNCH(ctx, "lc_partition", k_epoch_prep, dim3(cdiv((int64_t)a.nobs * B, 4)), dim3(256), 0,
+                 a.obs_off, a.nobs * B, a.nepoch, a.t, a.mobs, a.sig, ep, rng);
+  };
+  if (concurrent) run_beside(ctx, partition, *concurrent);
+  else partition();
   const size_t mapsz = (size_t)a.Nx * a.Ny;
   const size_t esz = a.precision == MLC_F64 ? 2 * sizeof(double) : 2 * sizeof(float);
   const bool force_general = ctx->tune_mode == 2 && (ctx->tune_force_flags & MLC_TUNE_GENERAL_NODES);
@@ -1605,10 +1618,12 @@ int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix, v
     // back while the row spectrum they all read stays in the last-level
     // cache (convolving each node right before the band launch that reads
     // it let the band kernels evict the spectrum between nodes, and the step
-    // took 3% longer)
+    // took 3% longer); the curve partition runs beside the node kernels
     NodeDriver drive = [&](const std::function<void(int)>& conv) {
-      for (int k = 0; k < nnodes; ++k) conv(k);
-      run_fused(ctx, a);
+      const std::function<void()> nodes = [&] {
+        for (int k = 0; k < nnodes; ++k) conv(k);
+      };
+      run_fused(ctx, a, nullptr, &nodes);
     };
     if (map->precision == MLC_F64) fft_convolve_nodes<double>(map, Ps, dst.data(), &drive);
     else fft_convolve_nodes<float>(map, Ps, dst.data(), &drive);
