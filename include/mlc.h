@@ -318,7 +318,11 @@ int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes,
  * layout mlc_map_convolve writes to d_node); node_shape: host array of
  * nnodes MLC_SHAPE_* values; node_rad doubles as each node's half-light
  * radius (in units of pix) for the convolution.  Every node map is written,
- * including those no band reads and when ncurves is 0. */
+ * including those no band reads and when ncurves is 0.  The curve
+ * partition runs on a second stream of the context (default priority,
+ * created on the first call that is not being captured) beside the node
+ * kernels, forked from and joined back into the context's stream by events
+ * inside the call. */
 int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix,
                               void* d_nodes, int nnodes, int P, int B,
                               const int* node_off, const int* node_cnt,
