@@ -125,15 +125,6 @@ __device__ __forceinline__ void row_fft(cpx<T>* buf, const Plan& plan, const TW&
   else dev_fft<T, G, 16, INV>(buf, plan, tw, pad, g);
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; this numbering gives
-// each XCD a contiguous range of logical blocks.  The grid is launched with
-// 8*ceil(nb/8) workgroups; ids >= nb return at once.
-__device__ __forceinline__ int xcd_block(int nb) {
-  const int per = (nb + 7) >> 3;
-  const int b = blockIdx.x;
-  return (b & 7) * per + (b >> 3);
-}
-
 // A store into a node map by the fused kernels, which write 64 or 128
 // contiguous bytes of a tile per workgroup.  Non-temporal: node maps are
 // written once per step and read by a later kernel, so they should not push

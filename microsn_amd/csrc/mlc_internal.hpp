@@ -270,4 +270,13 @@ int guarded(mlc_ctx* ctx, F&& f) {
   }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; this numbering gives
+// each XCD a contiguous range of logical blocks.  The grid is launched with
+// 8*ceil(nb/8) workgroups; ids >= nb return at once.
+__device__ __forceinline__ int xcd_block(int nb) {
+  const int per = (nb + 7) >> 3;
+  const int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
+}
+
 }  // namespace mlc

@@ -37,6 +37,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int64_t kSmallSort = 1 << 21;  // up to this many keys: segment sorts and merges
+constexpr int kScatterRounds = 16;       // rounds of a radix pass's tile (256 keys each)
 
 // True when every key of the pass has the same digit (block-uniform).
 __device__ __forceinline__ bool pass_trivial(const unsigned* __restrict__ hist_pass, int64_t n) {
@@ -59,27 +60,57 @@ __global__ void k_zero_u32(unsigned* __restrict__ p, int n) {
   if (i < n) p[i] = 0u;
 }
 
+// h[d] += 1 for the digit d of every active lane.  The digits of the first
+// two distinct values among the wave's lanes are added with one atomic each
+// (the high digits of keys from a narrow range take one or two values, and
+// 64 lanes on one LDS address serialise), the rest one by one.
+__device__ __forceinline__ void wave_count(unsigned* h, unsigned d) {
+  const int lane = threadIdx.x & 63;
+  bool done = false;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const uint64_t act = __ballot(!done);
+    if (act == 0) break;  // uniform
+    const int fl = __builtin_ctzll(act);
+    const unsigned dv = __shfl(d, fl, 64);
+    const uint64_t m = __ballot(!done && d == dv);
+    if (lane == fl) atomicAdd(&h[dv], (unsigned)__popcll(m));
+    done |= d == dv;
+  }
+  if (!done) atomicAdd(&h[d], 1u);
+}
+
 // Transform keys and histogram all eight digits.  hist: [8][256] uint32.
-__global__ __launch_bounds__(kThreads) void k_transform_hist(const double* __restrict__ in,
-                                                             uint64_t* __restrict__ out, int64_t n,
-                                                             unsigned* __restrict__ hist) {
+constexpr int kHistThreads = 1024;
+__global__ __launch_bounds__(kHistThreads) void k_transform_hist(const double* __restrict__ in,
+                                                                 uint64_t* __restrict__ out, int64_t n,
+                                                                 unsigned* __restrict__ hist) {
   __shared__ unsigned h[8][256];
-  for (int i = threadIdx.x; i < 8 * 256; i += kThreads) (&h[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < 8 * 256; i += kHistThreads) (&h[0][0])[i] = 0;
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kThreads) {
+  for (int64_t i = (int64_t)blockIdx.x * kHistThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kHistThreads) {
     uint64_t k = key_of(in[i]);
     out[i] = k;
+    // the low six digits are close to uniform: plain atomics; the sign and
+    // exponent digits often take one or two values
 #pragma unroll
-    for (int p = 0; p < 8; ++p) atomicAdd(&h[p][digit_of(k, p)], 1u);
+    for (int p = 0; p < 6; ++p) atomicAdd(&h[p][digit_of(k, p)], 1u);
+#pragma unroll
+    for (int p = 6; p < 8; ++p) wave_count(h[p], digit_of(k, p));
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * 256; i += kThreads) {
+  for (int i = threadIdx.x; i < 8 * 256; i += kHistThreads) {
     unsigned v = (&h[0][0])[i];
     if (v) atomicAdd(&hist[i], v);
   }
 }
 
+// Per-tile digit counts, digit-major.  Persistent, with the tiles split over
+// the XCDs as k_scatter splits them (grid a multiple of 8): the counts of
+// neighbouring tiles, which share cache lines of a digit's row, are
+// completed in one L2.  The next tile's keys are loaded while the current
+// tile's are counted.
 template <int R>
 __global__ __launch_bounds__(kThreads) void k_tile_hist(const uint64_t* __restrict__ keys, int64_t n,
                                                         int pass, int ntiles,
@@ -87,28 +118,76 @@ __global__ __launch_bounds__(kThreads) void k_tile_hist(const uint64_t* __restri
                                                         const unsigned* __restrict__ hist_pass) {
   if (pass_trivial(hist_pass, n)) return;
   __shared__ unsigned h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * (kThreads * R);
+  const int per = (ntiles + 7) >> 3, xcd = blockIdx.x & 7, nwx = gridDim.x >> 3;
+  const int tend = min(ntiles, (xcd + 1) * per);
+  int tile = xcd * per + (int)(blockIdx.x >> 3);
+  uint64_t kr[R];  // raw keys of the next tile: used only once they are needed
+  auto load = [&](int tl) {
+    const int64_t base = (int64_t)tl * (kThreads * R);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    int64_t i = base + r * kThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[digit_of(keys[i], pass)], 1u);
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = base + r * kThreads + threadIdx.x;
+      kr[r] = i < n ? keys[i] : 0ull;
+    }
+  };
+  if (tile < tend) load(tile);
+  for (; tile < tend; tile += nwx) {  // uniform per workgroup
+    const int64_t base = (int64_t)tile * (kThreads * R);
+    unsigned d[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r] = base + r * kThreads + threadIdx.x < n ? digit_of(kr[r], pass) : 256u;
+    if (tile + nwx < tend) load(tile + nwx);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    if (pass >= 6) {  // uniform: the sign and exponent digits
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (d[r] < 256u) wave_count(h, d[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (d[r] < 256u) atomicAdd(&h[d[r]], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * ntiles + tile] = h[threadIdx.x];
+    __syncthreads();  // h is cleared for the next tile
   }
-  __syncthreads();
-  counts[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
 // One workgroup per digit d: exclusive scan of the digit's tile counts
 // (each thread sums a contiguous run of tiles, the 256 run totals are
 // scanned in LDS) plus the digit's base, the number of keys with a smaller
-// digit (a block reduction over the pass histogram).
+// digit (a block reduction over the pass histogram).  With `staged`, the
+// digit's row of counts is first copied into LDS (dynamic shared memory of
+// ntiles words) with coalesced loads, scanned there and copied back, instead
+// of each thread walking its run of the row in global memory.
+constexpr int kScanStageTiles = 32768;  // rows up to this long are staged (128 KB)
 __global__ __launch_bounds__(kThreads) void k_scan_tiles(unsigned* __restrict__ counts, int ntiles,
                                                          const unsigned* __restrict__ hist_pass,
-                                                         int64_t n) {
+                                                         int64_t n, int staged) {
   if (pass_trivial(hist_pass, n)) return;
+  extern __shared__ unsigned rowl[];
   __shared__ unsigned s[kThreads];
   const int d = blockIdx.x, t = threadIdx.x;
+  unsigned* row = counts + (size_t)d * ntiles;
+  unsigned* src = row;
+  if (staged) {
+    constexpr int kBatch = 8;
+    for (int i0 = 0; i0 < ntiles; i0 += kBatch * kThreads) {
+      unsigned v[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const int i = i0 + k * kThreads + t;
+        v[k] = i < ntiles ? row[i] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const int i = i0 + k * kThreads + t;
+        if (i < ntiles) rowl[i] = v[k];
+      }
+    }
+    src = rowl;
+  }
   // digit base
   s[t] = t < d ? hist_pass[t] : 0u;
   __syncthreads();
@@ -118,11 +197,10 @@ __global__ __launch_bounds__(kThreads) void k_scan_tiles(unsigned* __restrict__ 
   }
   const unsigned base = s[0];
   __syncthreads();
-  unsigned* row = counts + (size_t)d * ntiles;
   const int seg = (ntiles + kThreads - 1) / kThreads;
   const int a = min(ntiles, t * seg), e = min(ntiles, a + seg);
   unsigned sum = 0;
-  for (int i = a; i < e; ++i) sum += row[i];
+  for (int i = a; i < e; ++i) sum += src[i];
   s[t] = sum;
   __syncthreads();
   for (int off = 1; off < kThreads; off <<= 1) {  // inclusive Hillis-Steele
@@ -133,9 +211,13 @@ __global__ __launch_bounds__(kThreads) void k_scan_tiles(unsigned* __restrict__ 
   }
   unsigned run = base + s[t] - sum;
   for (int i = a; i < e; ++i) {
-    const unsigned v = row[i];
-    row[i] = run;
+    const unsigned v = src[i];
+    src[i] = run;
     run += v;
+  }
+  if (staged) {
+    __syncthreads();
+    for (int i = t; i < ntiles; i += kThreads) row[i] = rowl[i];
   }
 }
 
@@ -148,8 +230,16 @@ __global__ __launch_bounds__(kThreads) void k_scan_tiles(unsigned* __restrict__ 
 // any is staged.  The counts are 16-bit (a tile holds 256 R <= 4096 keys)
 // and share LDS with the staged keys, which are written only after every
 // slot is known: 50 KB of LDS, three workgroups per CU.
+//
+// Persistent: XCD x (blockIdx & 7) owns the contiguous tiles [x per, (x+1)
+// per), and its workgroups take them in turn, so the tiles in flight on one
+// XCD are neighbours: the runs that neighbouring tiles write for one digit
+// are neighbours in kout, and the cache lines they share are completed in
+// that XCD's L2.  A workgroup loads the keys and payload of its next tile
+// into registers once the current tile's are staged in LDS, so the loads
+// overlap the current tile's writes.  The grid is a multiple of 8.
 template <int R>
-__global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict__ kin,
+__global__ __launch_bounds__(kThreads, 3) void k_scatter(const uint64_t* __restrict__ kin,
                                                       const unsigned* __restrict__ pin,
                                                       uint64_t* __restrict__ kout,
                                                       unsigned* __restrict__ pout, int64_t n,
@@ -159,14 +249,20 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
   constexpr int kRounds = R, kTile = kThreads * R;
   static_assert(kTile <= 65535, "16-bit tile ranks");
   static_assert(kThreads == 256, "one thread per digit");
+  const int per = (ntiles + 7) >> 3, xcd = blockIdx.x & 7, nwx = gridDim.x >> 3;
+  const int tend = min(ntiles, (xcd + 1) * per);
+  int tile = xcd * per + (int)(blockIdx.x >> 3);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (pass_trivial(hist_pass, n)) {
-    const int64_t base = (int64_t)blockIdx.x * kTile;
+    for (; tile < tend; tile += nwx) {
+      const int64_t base = (int64_t)tile * kTile;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t i = base + r * kThreads + threadIdx.x;
-      if (i < n) {
-        kout[i] = kin[i];
-        pout[i] = pin ? pin[i] : (unsigned)i;
+      for (int r = 0; r < R; ++r) {
+        const int64_t i = base + r * kThreads + t;
+        if (i < n) {
+          kout[i] = kin[i];
+          pout[i] = pin ? pin[i] : (unsigned)i;
+        }
       }
     }
     return;
@@ -180,86 +276,98 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* __restrict
   __shared__ unsigned wsum[kWaves];
   __shared__ unsigned start[256];  // first LDS slot of each digit (exclusive scan)
   __shared__ unsigned goff[256];   // where this tile's run of each digit starts in kout
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  for (int i = t; i < kRounds * kWaves * 256 / 8; i += kThreads)
-    reinterpret_cast<uint4*>(sk)[i] = make_uint4(0u, 0u, 0u, 0u);
-  goff[t] = offsets[(size_t)t * ntiles + blockIdx.x];
+  // the keys, payload and run offsets of the tile to rank next
   uint64_t key[kRounds];
-  unsigned pay[kRounds], dig[kRounds], slot[kRounds];
+  unsigned pay[kRounds], on = 0;
+  auto load = [&](int tl) {
+    const int64_t base = (int64_t)tl * kTile;
 #pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
-    const int64_t i = base + r * kThreads + t;
-    const bool ok = i < n;
-    key[r] = ok ? kin[i] : ~0ull;
-    pay[r] = ok ? (pin ? pin[i] : (unsigned)i) : 0u;
-    dig[r] = ok ? digit_of(key[r], pass) : 256u;  // 256: padding, not ranked
-  }
-  __syncthreads();
-  // rank within the wave: lanes of this wave holding the same digit
-  // (padding lanes match nobody); the first of them records the count
+    for (int r = 0; r < kRounds; ++r) {
+      const int64_t i = base + r * kThreads + t;
+      const bool ok = i < n;
+      key[r] = ok ? kin[i] : ~0ull;
+      pay[r] = ok ? (pin ? pin[i] : (unsigned)i) : 0u;
+    }
+    on = offsets[(size_t)t * ntiles + tl];
+  };
+  if (tile < tend) load(tile);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (; tile < tend; tile += nwx) {  // uniform per workgroup
+    const int64_t base = (int64_t)tile * kTile;
+    unsigned dig[kRounds], slot[kRounds];
 #pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
-    const unsigned d = dig[r];
-    uint64_t peers = __ballot(d < 256u);
+    for (int r = 0; r < kRounds; ++r)
+      dig[r] = base + r * kThreads + t < n ? digit_of(key[r], pass) : 256u;  // 256: padding, not ranked
+    goff[t] = on;
+    for (int i = t; i < kRounds * kWaves * 256 / 8; i += kThreads)
+      reinterpret_cast<uint4*>(sk)[i] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    // rank within the wave: lanes of this wave holding the same digit
+    // (padding lanes match nobody); the first of them records the count
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t m = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? m : ~m;
+    for (int r = 0; r < kRounds; ++r) {
+      const unsigned d = dig[r];
+      uint64_t peers = __ballot(d < 256u);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint64_t m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+      }
+      if (d >= 256u) peers = 0;
+      slot[r] = (unsigned)__popcll(peers & lt);
+      if (d < 256u && slot[r] == 0) cnt[r][w][d] = (unsigned short)__popcll(peers);
     }
-    if (d >= 256u) peers = 0;
-    slot[r] = (unsigned)__popcll(peers & lt);
-    if (d < 256u && slot[r] == 0) cnt[r][w][d] = (unsigned short)__popcll(peers);
-  }
-  __syncthreads();
-  // thread t owns digit t: exclusive prefix over (round, wave), then the
-  // digit's start from a scan of the totals over the digits
-  unsigned tot = 0;
+    __syncthreads();
+    // thread t owns digit t: exclusive prefix over (round, wave), then the
+    // digit's start from a scan of the totals over the digits
+    unsigned tot = 0;
 #pragma unroll
-  for (int r = 0; r < kRounds; ++r)
+    for (int r = 0; r < kRounds; ++r)
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-      const unsigned v = cnt[r][ww][t];
-      cnt[r][ww][t] = (unsigned short)tot;
-      tot += v;
+      for (int ww = 0; ww < kWaves; ++ww) {
+        const unsigned v = cnt[r][ww][t];
+        cnt[r][ww][t] = (unsigned short)tot;
+        tot += v;
+      }
+    unsigned incl = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
     }
-  unsigned incl = tot;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    unsigned pre = 0;
+    for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+    start[t] = pre + incl - tot;
+    __syncthreads();
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned u = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += u;
-  }
-  if (lane == 63) wsum[w] = incl;
-  __syncthreads();
-  unsigned pre = 0;
-  for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
-  start[t] = pre + incl - tot;
-  __syncthreads();
+    for (int r = 0; r < kRounds; ++r)
+      if (dig[r] < 256u) slot[r] += start[dig[r]] + cnt[r][w][dig[r]];
+    __syncthreads();  // every slot read its counts: the space now takes the keys
 #pragma unroll
-  for (int r = 0; r < kRounds; ++r)
-    if (dig[r] < 256u) slot[r] += start[dig[r]] + cnt[r][w][dig[r]];
-  __syncthreads();  // every slot read its counts: the space now takes the keys
-#pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
-    if (dig[r] < 256u) {
-      sk[slot[r]] = key[r];
-      sp[slot[r]] = pay[r];
+    for (int r = 0; r < kRounds; ++r) {
+      if (dig[r] < 256u) {
+        sk[slot[r]] = key[r];
+        sp[slot[r]] = pay[r];
+      }
     }
-  }
-  __syncthreads();
-  // out in runs that are contiguous in memory
-  const int valid = (int)min<int64_t>(kTile, n - base);
+    __syncthreads();
+    if (tile + nwx < tend) load(tile + nwx);  // this tile's keys are in LDS now
+    // out in runs that are contiguous in memory
+    const int valid = (int)min<int64_t>(kTile, n - base);
 #pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
-    const int q = r * kThreads + t;
-    if (q < valid) {
-      const uint64_t k = sk[q];
-      const unsigned d = digit_of(k, pass);
-      const size_t dst = (size_t)goff[d] + (unsigned)(q - start[d]);
-      kout[dst] = k;
-      pout[dst] = sp[q];
+    for (int r = 0; r < kRounds; ++r) {
+      const int q = r * kThreads + t;
+      if (q < valid) {
+        const uint64_t k = sk[q];
+        const unsigned d = digit_of(k, pass);
+        const size_t dst = (size_t)goff[d] + (unsigned)(q - start[d]);
+        kout[dst] = k;
+        pout[dst] = sp[q];
+      }
     }
+    __syncthreads();  // the next tile's counts overwrite sk and goff
   }
 }
 
@@ -591,6 +699,19 @@ const uint64_t* sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* 
                             unsigned* cnt, const unsigned* hist, int64_t n) {
   constexpr int kTile = kThreads * R;
   const int ntiles = (int)((n + kTile - 1) / kTile);
+  const int staged = ntiles <= kScanStageTiles;
+  const size_t scan_lds = staged ? sizeof(unsigned) * ntiles : 0;
+  if (scan_lds > 64 * 1024)
+    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_tiles),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)scan_lds),
+              "hipFuncSetAttribute(scan LDS)");
+  // persistent tile counts: eight workgroups per CU, a multiple of 8
+  const int hist_grid = (int)std::min<int64_t>(cdiv(ntiles, 8) * 8, cdiv(fill_grid(ctx, 8), 8) * 8);
+  // persistent scatter: the workgroups the CUs hold at once, a multiple of 8
+  const int scatter_grid =
+      (int)std::min<int64_t>(cdiv(ntiles, 8) * 8,
+                             cdiv(fill_grid(ctx, resident_blocks(reinterpret_cast<const void*>(&k_scatter<R>),
+                                                                 kThreads)), 8) * 8);
   // eight passes (trivial ones copy): ping-pong so that the last one writes perm
   unsigned* pa = pbuf;  // payload written by passes 0, 2, 4, 6
   unsigned* pb = perm;  // payload written by passes 1, 3, 5, 7
@@ -600,11 +721,12 @@ const uint64_t* sort_passes(mlc_ctx* ctx, uint64_t* k0, uint64_t* k1, unsigned* 
   for (int p = 0; p < 8; ++p) {
     unsigned* pout = (p % 2 == 0) ? pa : pb;
     const unsigned* hp = hist + p * 256;
-    MLC_LAUNCH(ctx, "sort_tile_hist", k_tile_hist<R>, dim3(ntiles), dim3(kThreads), 0, kin, n, p,
-               ntiles, cnt, hp);
-    MLC_LAUNCH(ctx, "sort_scan_tiles", k_scan_tiles, dim3(256), dim3(kThreads), 0, cnt, ntiles, hp, n);
-    MLC_LAUNCH(ctx, "sort_scatter", k_scatter<R>, dim3(ntiles), dim3(kThreads), 0, kin, pin, kout,
-               pout, n, p, ntiles, cnt, hp);
+    MLC_LAUNCH(ctx, "sort_tile_hist", k_tile_hist<R>, dim3(hist_grid), dim3(kThreads), 0, kin, n, p, ntiles,
+               cnt, hp);
+    MLC_LAUNCH(ctx, "sort_scan_tiles", k_scan_tiles, dim3(256), dim3(kThreads), scan_lds, cnt, ntiles, hp, n,
+               staged);
+    MLC_LAUNCH(ctx, "sort_scatter", k_scatter<R>, dim3(scatter_grid), dim3(kThreads), 0, kin, pin, kout, pout,
+               n, p, ntiles, cnt, hp);
     const uint64_t* nk = kout;
     kout = (kout == k1) ? k0 : k1;
     kin = nk;
@@ -624,7 +746,7 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   if (n <= 0) return;
   MLC_REQUIRE(n < ((int64_t)1 << 32), "sort: more than 2^32-1 keys");
   const bool small = n <= kSmallSort;
-  const int64_t tile = kThreads * 16;
+  const int64_t tile = kThreads * kScatterRounds;
   const int64_t ntiles = small ? 1 : (n + tile - 1) / tile;
   Workspace ws(ctx);
   int ik0 = ws.add(sizeof(uint64_t) * n), ik1 = ws.add(sizeof(uint64_t) * n);
@@ -654,10 +776,10 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
   } else {
     // a fill kernel, not a memset node, so that a captured graph replays it
     MLC_LAUNCH(ctx, "sort_zero_hist", k_zero_u32, dim3(8), dim3(256), 0, hist, 8 * 256);
-    const int g = (int)std::min<int64_t>(cdiv(n, kThreads * 4), fill_grid(ctx, 1));
-    MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kThreads), 0, keys, k0, n,
+    const int g = (int)std::min<int64_t>(cdiv(n, kHistThreads * 4), fill_grid(ctx, 2));
+    MLC_LAUNCH(ctx, "sort_transform_hist", k_transform_hist, dim3(g), dim3(kHistThreads), 0, keys, k0, n,
                hist);
-    skeys = sort_passes<16>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
+    skeys = sort_passes<kScatterRounds>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
   }
   if (rank || sorted)
     MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
