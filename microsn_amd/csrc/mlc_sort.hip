@@ -38,6 +38,7 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int64_t kSmallSort = 1 << 21;  // up to this many keys: segment sorts and merges
 constexpr int kScatterRounds = 16;       // rounds of a radix pass's tile (256 keys each)
+constexpr int64_t kRankWindow = (int64_t)32 << 20;  // ranks per scatter window (128 MB)
 
 // True when every key of the pass has the same digit (block-uniform).
 __device__ __forceinline__ bool pass_trivial(const unsigned* __restrict__ hist_pass, int64_t n) {
@@ -601,13 +602,14 @@ __device__ __forceinline__ double double_of(uint64_t k) {
 // from the sorted ordered keys skeys (read in order); only a key that does
 // not determine its double (a zero, whose sign it dropped, or a NaN, whose
 // payload it dropped) is gathered from the original keys.
+// Only ranks of the pairs p in [plo, phi) are written (see sort_f64).
 __global__ void k_rank_and_gather(const unsigned* __restrict__ perm, const uint64_t* __restrict__ skeys,
                                   const double* __restrict__ keys, int64_t n, unsigned* __restrict__ rank,
-                                  double* __restrict__ sorted) {
+                                  double* __restrict__ sorted, unsigned plo, unsigned phi) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const unsigned p = perm[i];
-    if (rank) rank[p] = (unsigned)i;
+    if (rank && p - plo < phi - plo) rank[p] = (unsigned)i;
     if (sorted) {
       const uint64_t k = skeys[i];
       sorted[i] = (k == ~0ull || k == 0x8000000000000000ull) ? keys[p] : double_of(k);
@@ -781,10 +783,18 @@ void sort_f64(mlc_ctx* ctx, const double* keys, int64_t n, unsigned* perm, unsig
                hist);
     skeys = sort_passes<kScatterRounds>(ctx, k0, k1, pbuf, perm, cnt, hist, n);
   }
-  if (rank || sorted)
-    MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
-               dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, skeys,
-               keys, n, rank, sorted);
+  if (rank || sorted) {
+    // The rank scatter writes 4 bytes to random places.  Over an array larger
+    // than the last-level cache every such write reaches HBM as a partial
+    // line; in windows of kRankWindow ranks, each launch reading all of perm
+    // but writing only into its window, the partial lines of a window meet
+    // in the cache first.
+    const int64_t win = rank ? kRankWindow : n;
+    for (int64_t lo = 0; lo < n; lo += win)
+      MLC_LAUNCH(ctx, "sort_rank_gather", k_rank_and_gather,
+                 dim3(std::min<int64_t>(cdiv(n, 256), fill_grid(ctx, 16))), dim3(256), 0, perm, skeys, keys, n,
+                 rank, lo == 0 ? sorted : nullptr, (unsigned)lo, (unsigned)std::min<int64_t>(n, lo + win));
+  }
 }
 
 // The bin table on the device: d_lower / d_npb from the upper bounds, no
