@@ -110,6 +110,17 @@ def algorithmic_bytes(name, esz, ncurves):
     }.get(name)
 
 
+# What bounds the two dominant kernels below the HBM roofline, from their
+# counters (profiles/counters_4096_f32.md).
+LIMITER = {
+    "lc_chi2_grad": "L1-to-L2 read requests: nearly every lane's tap load misses L1 "
+                    "(about 0.65 L2 requests per sample, 0.14 per clock per CU at 370 clocks of latency); "
+                    "its VALU alone is about half the launch",
+    "conv_node_fused": "filter phase: L1-to-L2 read requests (0.13 per clock per CU); FFT phase: VALU "
+                       "(no packed FP32)",
+}
+
+
 def tap_bytes(esz, ncurves):
     """Bytes of bilinear taps one chi2 launch gathers: 4 (mu, dmu/dr) taps per
     sample (L2/L1 traffic, not HBM)."""
@@ -858,6 +869,8 @@ def main():
         if k == "lc_chi2_grad":  # its gathers, served by the caches
             r["tap_bytes_per_launch"] = tap_bytes(4, ncurves)
             r["tap_rate_GBps"] = tap_bytes(4, ncurves) / avg_s / 1e9
+        if k in LIMITER:
+            r["limiter"] = LIMITER[k]
         return r
 
     # the dominant kernel by time per step, and every kernel with algorithmic bytes
