@@ -431,6 +431,28 @@ def time_batch_workload(ctx, which, steps):
                                  "unit": "GB/s", "frac": ab / (ms / 1e3) / HBM_PEAK,
                                  "algorithmic_bytes_per_launch": ab / launches, "avg_launch_ms": ms / launches,
                                  "tap_bytes_per_step": taps, "tap_rate_GBps": taps / (ms / 1e3) / 1e9}}
+    # the dominant kernel's roofline over the step (FP32 maps: 8-byte spectrum
+    # bins and 8-byte (mu, dmu/dr) pixels; the map transform reads FP64)
+    def step_bytes(name):
+        tot = 0
+        for m, ns in zip(w["maps"], w["nodesets"]):
+            ny, nx = int(m.shape[0]), int(m.shape[1])
+            spec, px, nn = ny * (nx // 2 + 1) * 8, ny * nx * 8, int(ns.nodes.shape[0])
+            per = {"fft_row_c2r_node": nn * (2 * spec + px), "conv_node_fused": nn * (spec + px),
+                   "conv_node_yfilter": nn * 3 * spec, "fft_row_r2c": ny * nx * 8 + spec}.get(name)
+            if per is None:
+                return None
+            tot += per
+        return tot
+    if dom == "lc_batch":
+        rec["dominant_roofline"] = dict(rec["lc_batch_roofline"], kernel=dom)
+    elif step_bytes(dom):
+        dms, dl = kern[dom]
+        rec["dominant_roofline"] = {"kernel": dom, "bound": "hbm", "achieved": step_bytes(dom) / (dms / 1e3) / 1e9,
+                                    "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                    "frac": step_bytes(dom) / (dms / 1e3) / HBM_PEAK,
+                                    "algorithmic_bytes_per_step": step_bytes(dom), "launches_per_step": dl,
+                                    "ms_per_step": dms}
     for sp in w["specs"]:
         sp.close()
     del w, eager, out, g
