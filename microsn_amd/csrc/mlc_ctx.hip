@@ -109,13 +109,59 @@ __global__ void k_sample_points(const T* __restrict__ conv, int Nx, int Ny, int 
 }
 
 namespace mlc {
+void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(ctx->stream, &st), "hipStreamIsCapturing");
+  if (n > 1 && !ctx->spread[0] && st == hipStreamCaptureStatusNone) {
+    if (!ctx->fork_ev)
+      hip_check(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)");
+    for (int i = 0; i < mlc_ctx::kSpread; ++i) {
+      // default priority, like every other stream of the library
+      hip_check(hipStreamCreateWithFlags(&ctx->spread[i], hipStreamNonBlocking), "hipStreamCreateWithFlags(spread)");
+      hip_check(hipEventCreateWithFlags(&ctx->spread_ev[i], hipEventDisableTiming), "hipEventCreate(spread)");
+    }
+  }
+  // with per-kernel timing on, in order: the kernels' own durations, not
+  // those of kernels sharing the chip
+  if (n <= 1 || !ctx->spread[0] || ctx->timing) {
+    for (int i = 0; i < n; ++i) work(i);
+    return;
+  }
+  const hipStream_t main = ctx->stream;
+  const int ns = std::min(n, (int)mlc_ctx::kSpread);
+  hip_check(hipEventRecord(ctx->fork_ev, main), "hipEventRecord(fork)");
+  for (int s = 0; s < ns; ++s) hip_check(hipStreamWaitEvent(ctx->spread[s], ctx->fork_ev, 0), "hipStreamWaitEvent(fork)");
+  auto join = [&] {
+    ctx->stream = main;
+    for (int s = 0; s < ns; ++s) {
+      (void)hipEventRecord(ctx->spread_ev[s], ctx->spread[s]);
+      (void)hipStreamWaitEvent(main, ctx->spread_ev[s], 0);
+    }
+  };
+  try {
+    for (int i = 0; i < n; ++i) {
+      ctx->stream = ctx->spread[i % ns];
+      work(i);
+    }
+  } catch (...) {
+    join();
+    throw;
+  }
+  ctx->stream = main;
+  for (int s = 0; s < ns; ++s) {
+    hip_check(hipEventRecord(ctx->spread_ev[s], ctx->spread[s]), "hipEventRecord(spread)");
+    hip_check(hipStreamWaitEvent(main, ctx->spread_ev[s], 0), "hipStreamWaitEvent(spread)");
+  }
+}
+
 void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::function<void()>& main_work) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   hip_check(hipStreamIsCapturing(ctx->stream, &st), "hipStreamIsCapturing");
   if (!ctx->side && st == hipStreamCaptureStatusNone) {
     // default priority, like every other stream of the library
     hip_check(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreateWithFlags(side)");
-    hip_check(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)");
+    if (!ctx->fork_ev)
+      hip_check(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)");
     hip_check(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming), "hipEventCreate(join)");
   }
   if (!ctx->side) {
@@ -205,6 +251,13 @@ void mlc_ctx_destroy(mlc_ctx* ctx) {
   }
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+  for (int i = 0; i < mlc_ctx::kSpread; ++i) {
+    if (ctx->spread[i]) {
+      (void)hipStreamSynchronize(ctx->spread[i]);
+      (void)hipStreamDestroy(ctx->spread[i]);
+    }
+    if (ctx->spread_ev[i]) (void)hipEventDestroy(ctx->spread_ev[i]);
+  }
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete ctx;

@@ -96,6 +96,11 @@ struct mlc_ctx {
   // first use, outside graph capture)
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // more streams (default priority) for independent per-map work of one call
+  // (run_spread), with a join event each, created like `side`
+  static constexpr int kSpread = 4;
+  hipStream_t spread[kSpread] = {};
+  hipEvent_t spread_ev[kSpread] = {};
 };
 
 namespace mlc {
@@ -108,6 +113,16 @@ namespace mlc {
 // then (and without it) the two run one after the other.  Kernels launched by
 // `beside` go to the side stream through ctx->stream.
 void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::function<void()>& main_work);
+
+// Runs work(i) for i = 0 .. n-1, item i on spread stream i mod kSpread: the
+// spread streams wait for the main stream's work so far, and the main stream
+// waits for all of them before anything issued after this call.  Items run
+// concurrently, so they must not share scratch memory (per-map work on
+// per-map buffers).  The streams are created on the first call that is not
+// being captured; until then, for n <= 1 and while per-kernel timing is on,
+// the items run in order on the main stream.  Under graph capture they
+// become parallel branches.
+void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work);
 
 // Makes `ctx`'s device current for the lifetime of the guard.
 struct DeviceGuard {
