@@ -165,8 +165,13 @@ int mlc_map_convolve_nodes(mlc_map* map, int count, const int* shape, const doub
  *     for every i, with map i's nodes at offsets counts[0] + ... +
  *     counts[i-1] of shape, rhalf, rin, a and d_nodes;
  * bit for bit, but the small per-map kernels (the mean and DC terms of the
- * updates; the profiles and cosine taps of the nodes) run once for all maps.
- * Host arrays of nmaps (counts: of sum counts) entries. */
+ * updates; the profiles and cosine taps of the nodes) run once for all maps,
+ * and when every map is at most 4096 x 4096 the maps' row transforms and
+ * node maps run side by side on four more streams of the context (default
+ * priority), forked from the context's stream and joined back to it by
+ * events: to the caller the call is ordered on that stream as before, and
+ * under graph capture the maps become parallel branches.  Host arrays of
+ * nmaps (counts: of sum counts) entries. */
 int mlc_maps_update(int nmaps, mlc_map* const* maps, const double* const* d_maps);
 int mlc_maps_convolve_nodes(int nmaps, mlc_map* const* maps, const int* counts, const int* shape,
                             const double* rhalf, double pix, const double* rin, const double* a,
