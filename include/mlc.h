@@ -170,8 +170,10 @@ int mlc_map_convolve_nodes(mlc_map* map, int count, const int* shape, const doub
  * node maps run side by side on four more streams of the context (default
  * priority), forked from the context's stream and joined back to it by
  * events: to the caller the call is ordered on that stream as before, and
- * under graph capture the maps become parallel branches.  Host arrays of
- * nmaps (counts: of sum counts) entries. */
+ * under graph capture the maps become parallel branches.  A map may be
+ * listed more than once: it ends as the per-map calls in list order leave
+ * it (updated from its last source; every listing's nodes convolved).  Host
+ * arrays of nmaps (counts: of sum counts) entries. */
 int mlc_maps_update(int nmaps, mlc_map* const* maps, const double* const* d_maps);
 int mlc_maps_convolve_nodes(int nmaps, mlc_map* const* maps, const int* counts, const int* shape,
                             const double* rhalf, double pix, const double* rin, const double* a,

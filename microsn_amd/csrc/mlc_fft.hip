@@ -151,9 +151,19 @@ int mlc_maps_update(int nmaps, mlc_map* const* maps, const double* const* d_maps
     check_maps(nmaps, maps, "mlc_maps_update");
     MLC_REQUIRE(d_maps, "mlc_maps_update: null source list");
     for (int i = 0; i < nmaps; ++i) MLC_REQUIRE(d_maps[i], "mlc_maps_update: null map");
-    std::vector<mlc_map*> ms(maps, maps + nmaps);
-    if (maps[0]->precision == MLC_F64) fft_maps_transform<double>(ms, d_maps);
-    else fft_maps_transform<float>(ms, d_maps);
+    // a map listed more than once ends as the per-map calls in list order
+    // leave it: updated from its last source
+    std::vector<mlc_map*> ms;
+    std::vector<const double*> src;
+    for (int i = 0; i < nmaps; ++i) {
+      bool later = false;
+      for (int j = i + 1; j < nmaps; ++j) later |= maps[j] == maps[i];
+      if (later) continue;
+      ms.push_back(maps[i]);
+      src.push_back(d_maps[i]);
+    }
+    if (maps[0]->precision == MLC_F64) fft_maps_transform<double>(ms, src.data());
+    else fft_maps_transform<float>(ms, src.data());
   });
 }
 
@@ -165,8 +175,12 @@ int mlc_maps_convolve_nodes(int nmaps, mlc_map* const* maps, const int* counts, 
   return guarded(maps[0]->ctx, [&] {
     check_maps(nmaps, maps, "mlc_maps_convolve_nodes");
     MLC_REQUIRE(counts, "mlc_maps_convolve_nodes: null counts");
-    std::vector<mlc_map*> ms(maps, maps + nmaps);
-    std::vector<std::vector<ProfileParams>> Ps(nmaps);
+    // one entry per distinct map, in order of first appearance, holding the
+    // nodes of every listing of it (a map's node tables live in its own
+    // storage, so each map is tabled once)
+    std::vector<mlc_map*> ms;
+    std::vector<std::vector<ProfileParams>> Ps;
+    std::vector<std::vector<void*>> dn;
     int off = 0;
     for (int i = 0; i < nmaps; ++i) {
       MLC_REQUIRE(counts[i] >= 0, "mlc_maps_convolve_nodes: negative count");
@@ -174,15 +188,26 @@ int mlc_maps_convolve_nodes(int nmaps, mlc_map* const* maps, const int* counts, 
       MLC_REQUIRE(shape && rhalf && d_nodes, "mlc_maps_convolve_nodes: null pointer");
       MLC_REQUIRE(maps[i]->Nx % 4 == 0 && maps[i]->Ny % 4 == 0,
                   "mlc_maps_convolve_nodes: node maps (4 x 4 tiles) need Nx and Ny divisible by 4");
+      size_t e = 0;
+      while (e < ms.size() && ms[e] != maps[i]) ++e;
+      if (e == ms.size()) {
+        ms.push_back(maps[i]);
+        Ps.emplace_back();
+        dn.emplace_back();
+      }
       for (int k = off; k < off + counts[i]; ++k) {
         MLC_REQUIRE(d_nodes[k], "mlc_maps_convolve_nodes: null node map");
-        Ps[i].push_back(make_params(shape[k], rhalf[k], pix, rin ? rin[k] : 0.0, a ? a[k] : 1.0, maps[i]->Nx,
+        Ps[e].push_back(make_params(shape[k], rhalf[k], pix, rin ? rin[k] : 0.0, a ? a[k] : 1.0, maps[i]->Nx,
                                     maps[i]->Ny));
+        dn[e].push_back(d_nodes[k]);
       }
       off += counts[i];
     }
-    if (maps[0]->precision == MLC_F64) fft_maps_convolve_nodes<double>(ms, Ps, d_nodes);
-    else fft_maps_convolve_nodes<float>(ms, Ps, d_nodes);
+    std::vector<void*> flat;
+    for (auto& v : dn) flat.insert(flat.end(), v.begin(), v.end());
+    if (ms.empty()) return;
+    if (maps[0]->precision == MLC_F64) fft_maps_convolve_nodes<double>(ms, Ps, flat.data());
+    else fft_maps_convolve_nodes<float>(ms, Ps, flat.data());
   });
 }
 

@@ -6,6 +6,7 @@ mean subtraction, one map without nodes, FP32 and FP64:
     mlc_map_convolve_nodes on each map) bit for bit;
   * a captured batched update + convolution replays bit-equal to the eager
     calls, with eager work between the replays;
+  * a map listed twice in one batched call gives the per-map calls' bits;
   * maps of different precisions or contexts, and a map of the wrong shape,
     are refused."""
 import numpy as np
@@ -65,6 +66,30 @@ def test_batched_map_calls_equal_per_map_calls(ctx, prec):
         mu = ns.rowmajor()[0, :, :, 0].double().mean()
         assert abs(float(mu) / float(maps1[i].mean()) - 1) < 1e-5, i
     for sp in spa + spb:
+        sp.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_batched_map_calls_with_a_map_listed_twice(ctx, prec):
+    """One spectrum behind two node sets (the y-filter path at 1200 x 600 and
+    the fused path at 256^2), and a spectrum updated twice in one call: the
+    same bits as the per-map calls (a map listed twice is not spread over
+    streams)."""
+    maps0, maps1 = _maps(31), _maps(32)
+    for idx in (4, 0):
+        sp = api.MapSpectrum(ctx, maps0[idx], prec)
+        a = lcm.NodeSet(ctx, sp, ["gaussian", "uniform"], [[[2.0, 4.5]], [[3.0]]], [1.0], compute=False)
+        b = lcm.NodeSet(ctx, sp, ["exponential", "gaussian"], [[[1.5, 6.0]], [[2.5]]], [1.0], compute=False)
+        print("map listed twice", prec, SIZES[idx], flush=True)
+        api.update_spectra([sp, sp], [maps1[idx], maps1[idx]])
+        lcm.NodeBatch([a, b]).compute()
+        torch.cuda.synchronize()
+        ra, rb = a.nodes.clone(), b.nodes.clone()
+        sp.update(maps1[idx])
+        a.compute()
+        b.compute()
+        torch.cuda.synchronize()
+        assert torch.equal(ra, a.nodes) and torch.equal(rb, b.nodes), idx
         sp.close()
 
 
