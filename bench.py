@@ -262,7 +262,7 @@ def trajectories(rng, n, size, length, tspan, margin):
     return np.stack([x0, y0, th, np.full(n, length / tspan), np.zeros(n), np.zeros(n), rng.normal(0, 0.05, n)], 1)
 
 
-def config3_workload(ctx, prec="f32", ncur=2048, seed=3003):
+def batch_64_maps_workload(ctx, prec="f32", ncur=2048, seed=3003):
     """64 maps (16 systems x 4 images) of 1024^2..4096^2, kappa and gamma
     uniform in 0.2..0.6 with (1-k)^2 - g^2 > 0.05 (map mean 1/|(1-k)^2-g^2|),
     flux ratios log-uniform in 0.1..1 as each image's magnitude offset, a
@@ -307,7 +307,7 @@ def config3_workload(ctx, prec="f32", ncur=2048, seed=3003):
                 obs_idx=t(oidx), bins=api.mpd_bins(100, 1.0, 1e7))
 
 
-def config4_workload(ctx, prec="f32", ntraj=1_000_000, seed=4004):
+def scale_10000_workload(ctx, prec="f32", ntraj=1_000_000, seed=4004):
     """Four 10000^2 maps (one per lensed image), uniform-disc sources of
     1, 2.5, 5, 10 and 20 px in five bands of 40 epochs each (200 per curve,
     0.02 mag noise), ntraj trajectories on every image (4 ntraj curves), a
@@ -381,7 +381,7 @@ def time_batch_workload(ctx, which, steps):
     batched band kernel's roofline from its event-timed launch."""
     import torch
 
-    w = config3_workload(ctx) if which == "config3" else config4_workload(ctx)
+    w = batch_64_maps_workload(ctx) if which == "batch_64_maps" else scale_10000_workload(ctx)
     torch.cuda.synchronize()
     batch_step(ctx, w)  # first step: launch shapes tuned, tables and workspace in place
     torch.cuda.synchronize()
@@ -544,7 +544,7 @@ def replay_ms(ctx, fn, steps):
     return (time.perf_counter() - t0) / steps * 1e3, same
 
 
-def time_config0(ctx, steps, cpu=True):
+def time_plain_1000(ctx, steps, cpu=True):
     """The CPU-runnable workload: 1000^2 map (log-normal, 3 px smoothing,
     mean 5), Gaussian source of sigma 5 px, 1000 trajectories of 200 px over
     100 days, 50 irregular epochs against one observed curve with 0.05 mag
@@ -618,7 +618,7 @@ def time_config0(ctx, steps, cpu=True):
     return rec
 
 
-def time_config2(ctx, steps):
+def time_photosphere_2048(ctx, steps):
     """The differentiation workload: 2048^2 map, expanding uniform-disc
     photosphere r(t) = r0 + w t (r0 = 1 px, w uniform in 0.05..0.5 px/day,
     12 radius nodes from 1 to 12.5 px), 16384 trajectories of 60 px over 20
@@ -792,12 +792,12 @@ def main():
         return
     if args.extras_only:
         ctx = api.Context(0)
-        print(json.dumps({"extras": {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
-                                     "config4_scale_10000": time_batch_workload(ctx, "config4",
+        print(json.dumps({"extras": {"batching_64_maps": time_batch_workload(ctx, "batch_64_maps", args.steps),
+                                     "scale_10000": time_batch_workload(ctx, "scale_10000",
                                                                                 max(3, args.steps // 4)),
                                      "pair_mode_grid_input": time_pair_mode(ctx),
-                                     "config0_1000": time_config0(ctx, args.steps, cpu=False),
-                                     "config2_photosphere_2048": time_config2(ctx, args.steps)}}))
+                                     "plain_1000": time_plain_1000(ctx, args.steps, cpu=False),
+                                     "photosphere_2048": time_photosphere_2048(ctx, args.steps)}}))
         return
     t_setup = time.perf_counter()
     m = make_map()
@@ -905,11 +905,11 @@ def main():
     if not args.no_extras:
         del ws32
         torch.cuda.empty_cache()
-        extras = {"config3_batching_64_maps": time_batch_workload(ctx, "config3", args.steps),
-                  "config4_scale_10000": time_batch_workload(ctx, "config4", max(3, args.steps // 4)),
+        extras = {"batching_64_maps": time_batch_workload(ctx, "batch_64_maps", args.steps),
+                  "scale_10000": time_batch_workload(ctx, "scale_10000", max(3, args.steps // 4)),
                   "pair_mode_grid_input": time_pair_mode(ctx),
-                  "config0_1000": time_config0(ctx, args.steps, cpu=not args.no_cpu),
-                  "config2_photosphere_2048": time_config2(ctx, args.steps)}
+                  "plain_1000": time_plain_1000(ctx, args.steps, cpu=not args.no_cpu),
+                  "photosphere_2048": time_photosphere_2048(ctx, args.steps)}
 
     cpu = None
     if not args.no_cpu:

@@ -49,7 +49,7 @@ def _inputs(seed=2001):
     return params, prof, times, mobs, sig
 
 
-def test_config1_full_shape(ctx, orc):
+def test_precision_4096_full_shape(ctx, orc):
     m = _map(ctx)
     params, prof, times, mobs, sig = _inputs()
     n = params.shape[0]
@@ -60,7 +60,7 @@ def test_config1_full_shape(ctx, orc):
     rr = [[[R0 * s] for s in BAND_SCALE] for _ in SHAPES]
     out = {}
     for prec in ("f32", "f64"):
-        print(f"config 1: {prec} fused path", flush=True)
+        print(f"precision 4096^2: {prec} fused path", flush=True)
         spec = api.MapSpectrum(ctx, m, prec)
         nodes = lcm.NodeSet(ctx, spec, SHAPES, rr, BAND_SCALE, compute=False)
         c, g = lcm.chi2_grad(ctx, nodes, obs, P, pr, oi, convolve=True)
@@ -73,7 +73,7 @@ def test_config1_full_shape(ctx, orc):
     fin = np.isfinite(c64)
     assert fin.all(), "every trajectory stays on the map"
     rel = np.abs(c32 - c64) / np.abs(c64)
-    print(f"config 1: FP32 vs FP64 chi2 max rel err {rel.max():.3e}", flush=True)
+    print(f"precision 4096^2: FP32 vs FP64 chi2 max rel err {rel.max():.3e}", flush=True)
     assert rel.max() <= 1e-5
     assert np.array_equal(out["f32"]["lo"], out["f64"]["lo"])
     assert np.array_equal(out["f32"]["npb"], out["f64"]["npb"])

@@ -1,25 +1,26 @@
-"""The workload configurations other than the benchmarked one, run at their
-own shapes as parity cases:
+"""The workloads other than the benchmarked one, run at their own shapes as
+parity cases:
 
-  config 0   1000^2 FP64 map, Gaussian source (sigma 5 px), 1000 trajectories
-             of 200 px, 50 irregular epochs, chi2 sorted into 100 bins:
-             FP64 and FP32 paths against the FP64 oracle, order and bins
-             bit-exact
-  config 2   2048^2 FP32 map, expanding photosphere r(t) = r0 + w t (r0 1 px,
-             w in 0.05..0.5 px/day), 16384 trajectories: the FP32 adjoint
-             against the FP64 adjoint, the FP64 adjoint against central
-             differences (h = 1e-3)
-  config 3   64 maps of 1024^2..4096^2 (16 systems x 4 images), 20..400
-             epochs per curve, all curves in one batched call; the whole
-             step captured as one HIP graph
-  config 4   10000^2 FP32 maps of 4 images x 5 bands with disc profiles of
-             1..20 px, 1e6 trajectories per image, 200 epochs, 1000-bin table
-             over all 4e6 chi2, the whole step captured as one HIP graph:
-             replays bit-equal to eager, then size-independent properties
-             (FP32 against FP64 on a sample, sortedness and stable order
-             against numpy, bin table against a host search of the sorted
-             keys)
-The workload builders are bench.py's (config3_workload, config4_workload).
+  plain 1000^2       1000^2 FP64 map, Gaussian source (sigma 5 px), 1000
+                     trajectories of 200 px, 50 irregular epochs, chi2 sorted
+                     into 100 bins: FP64 and FP32 paths against the FP64
+                     oracle, order and bins bit-exact
+  photosphere 2048^2 2048^2 FP32 map, expanding photosphere r(t) = r0 + w t
+                     (r0 1 px, w in 0.05..0.5 px/day), 16384 trajectories:
+                     the FP32 adjoint against the FP64 adjoint, the FP64
+                     adjoint against central differences (h = 1e-3)
+  batching 64 maps   64 maps of 1024^2..4096^2 (16 systems x 4 images),
+                     20..400 epochs per curve, all curves in one batched
+                     call; the whole step captured as one HIP graph
+  scale 10000^2      10000^2 FP32 maps of 4 images x 5 bands with disc
+                     profiles of 1..20 px, 1e6 trajectories per image, 200
+                     epochs, 1000-bin table over all 4e6 chi2, the whole step
+                     captured as one HIP graph: replays bit-equal to eager,
+                     then size-independent properties (FP32 against FP64 on a
+                     sample, sortedness and stable order against numpy, bin
+                     table against a host search of the sorted keys)
+The workload builders are bench.py's (batch_64_maps_workload,
+scale_10000_workload).
 """
 import numpy as np
 import pytest
@@ -57,7 +58,7 @@ def _bilinear(m, x, y):
             fy * ((1 - fx) * m[iy + 1, ix] + fx * m[iy + 1, ix + 1]))
 
 
-def test_config0_end_to_end(ctx, orc):
+def test_plain_1000_end_to_end(ctx, orc):
     N, n, E = 1000, 1000, 50
     rng = np.random.default_rng(1000)
     m = _lognormal_map(ctx, N, 1000).cpu().numpy()
@@ -89,7 +90,7 @@ def test_config0_end_to_end(ctx, orc):
         spec = api.MapSpectrum(ctx, torch.tensor(m, device="cuda"), prec)
         nodes = lcm.NodeSet(ctx, spec, ["gaussian"], [[[rhalf]]], [1.0])
         for plain in ((False, True) if prec == "f64" else (False,)):
-            print("config 0", prec, "plain" if plain else "fused", flush=True)
+            print("plain 1000^2", prec, "plain" if plain else "fused", flush=True)
             c, g = lcm.chi2_grad(ctx, nodes, obs, P, pr, pr, plain=plain)
             cc = c.cpu().numpy()
             assert np.max(np.abs(cc - rc) / rc) < tol
@@ -113,7 +114,7 @@ def test_config0_end_to_end(ctx, orc):
         spec.close()
 
 
-def test_config2_differentiation(ctx):
+def test_photosphere_2048_differentiation(ctx):
     N, n, E, T = 2048, 16384, 40, 20.0
     rng = np.random.default_rng(2002)
     m = _lognormal_map(ctx, N, 2002)
@@ -130,7 +131,7 @@ def test_config2_differentiation(ctx):
     for prec in ("f32", "f64"):
         spec = api.MapSpectrum(ctx, m, prec)
         nodes = lcm.NodeSet(ctx, spec, ["uniform"], [[radii]], [1.0])
-        print("config 2", prec, flush=True)
+        print("photosphere 2048^2", prec, flush=True)
         out[prec] = lcm.chi2_grad(ctx, nodes, obs, P, pr, pr)
         if prec == "f64":
             nodes64 = nodes
@@ -161,7 +162,7 @@ def test_config2_differentiation(ctx):
     assert np.quantile(err, 0.9) < 2e-2, np.quantile(err, 0.9, axis=1)
 
 
-def test_config4_scale_captured(ctx):
+def test_scale_10000_captured(ctx):
     """Four 10000^2 FP32 maps (one per image) x 5 bands of discs 1..20 px,
     1e6 trajectories per image, 200 epochs: map spectra, 20 node maps, the
     batched chi2 + adjoint of all 4e6 curves, their stable sort and a
@@ -170,11 +171,11 @@ def test_config4_scale_captured(ctx):
     numpy's stable argsort and the bin table against a host search."""
     import bench
 
-    w = bench.config4_workload(ctx)
-    print("config 4 eager step", flush=True)
+    w = bench.scale_10000_workload(ctx)
+    print("scale 10000^2 eager step", flush=True)
     eager = [x.clone() for x in bench.batch_step(ctx, w)]
     torch.cuda.synchronize()
-    print("config 4 capture", flush=True)
+    print("scale 10000^2 capture", flush=True)
     g, out, gen = bench.capture(ctx, lambda: bench.batch_step(ctx, w))
     for r in range(2):
         assert ctx.generation() == gen
@@ -199,7 +200,7 @@ def test_config4_scale_captured(ctx):
     del n64
     s64.close()
     host = c.cpu().numpy()
-    print("config 4 sort", host.size, flush=True)
+    print("scale 10000^2 sort", host.size, flush=True)
     want = np.argsort(host, kind="stable")
     assert np.array_equal(perm.cpu().numpy().astype(np.int64), want)
     sk_h = sk.cpu().numpy()
@@ -215,7 +216,7 @@ def test_config4_scale_captured(ctx):
         sp.close()
 
 
-def test_config3_batching_one_call(ctx):
+def test_batching_64_maps_one_call(ctx):
     """64 maps of 1024^2..4096^2 with kappa, gamma in 0.2..0.6 ((1-k)^2-g^2 >
     0.05), 16 systems of 4 images with flux ratios log-uniform in 0.1..1 (the
     per-image magnitude offset), light curves of 20..400 epochs, all curves
@@ -230,8 +231,8 @@ def test_config3_batching_one_call(ctx):
     try:
         res = {}
         for prec in ("f32", "f64"):
-            w = bench.config3_workload(ctx, prec)
-            print("config 3", prec, "maps", len(w["maps"]), flush=True)
+            w = bench.batch_64_maps_workload(ctx, prec)
+            print("batching 64 maps", prec, "maps", len(w["maps"]), flush=True)
             res[prec] = (w, [x.clone() for x in bench.batch_step(ctx, w)])
         w, out32 = res["f32"]
         _, out64 = res["f64"]
