@@ -722,7 +722,7 @@ def cpu_baseline(m, params, prof, times, mobs, sig, bins):
     ncurves = len(prof)
     return {"value": ncurves / t_step, "unit": "light curves/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-            "sample": (f"one whole step on the box's host with the FP64 oracle: 9 node maps as 18 "
+            "sample": (f"one whole step on this machine's host CPUs with the FP64 oracle: 9 node maps as 18 "
                        f"4096^2 FFT convolutions ({t_conv:.2f} s; each transforms the map again, "
                        f"54 2-D FFTs where 37 would do), chi2 + adjoint of {ncurves} curves x 300 "
                        f"epochs ({t_curves:.2f} s), stable sort + bin table ({t_sort:.2f} s); "
