@@ -772,6 +772,8 @@ def main():
     ap.add_argument("--pmc-step", action="store_true",
                     help="run two FP32 steps and a 512 MiB device copy (the byte-counter "
                          "calibration) for rocprofv3 --pmc, print nothing else")
+    ap.add_argument("--pair-only", action="store_true",
+                    help="run only the pair-mode workload, one timed round (for profiling it)")
     ap.add_argument("--pmc-lanes", type=int, default=16,
                     help="band launch shape of --pmc-step: lanes per curve")
     ap.add_argument("--pmc-grid", type=int, default=1,
@@ -789,6 +791,9 @@ def main():
 
     if args.pmc_step:
         pmc_step(args.pmc_lanes, args.pmc_grid)
+        return
+    if args.pair_only:
+        print(json.dumps({"extras": {"pair_mode_grid_input": time_pair_mode(api.Context(0), steps=1)}}))
         return
     if args.extras_only:
         ctx = api.Context(0)
