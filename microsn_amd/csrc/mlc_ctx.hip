@@ -109,7 +109,7 @@ __global__ void k_sample_points(const T* __restrict__ conv, int Nx, int Ny, int 
 }
 
 namespace mlc {
-void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work) {
+void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work, const std::vector<double>* weight) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   hip_check(hipStreamIsCapturing(ctx->stream, &st), "hipStreamIsCapturing");
   if (n > 1 && !ctx->spread[0] && st == hipStreamCaptureStatusNone) {
@@ -138,9 +138,25 @@ void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work) {
       (void)hipStreamWaitEvent(main, ctx->spread_ev[s], 0);
     }
   };
+  // the stream of each item: round robin, or with weights the heaviest
+  // items first, each to the stream with the least weight so far
+  std::vector<int> order(n), on(n);
+  for (int i = 0; i < n; ++i) {
+    order[i] = i;
+    on[i] = i % ns;
+  }
+  if (weight && (int)weight->size() == n) {
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return (*weight)[x] > (*weight)[y]; });
+    std::vector<double> load(ns, 0.0);
+    for (int i : order) {
+      const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      on[i] = s;
+      load[s] += (*weight)[i];
+    }
+  }
   try {
-    for (int i = 0; i < n; ++i) {
-      ctx->stream = ctx->spread[i % ns];
+    for (int i : order) {
+      ctx->stream = ctx->spread[on[i]];
       work(i);
     }
   } catch (...) {

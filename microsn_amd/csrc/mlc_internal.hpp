@@ -114,7 +114,9 @@ namespace mlc {
 // `beside` go to the side stream through ctx->stream.
 void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::function<void()>& main_work);
 
-// Runs work(i) for i = 0 .. n-1, item i on spread stream i mod kSpread: the
+// Runs work(i) for i = 0 .. n-1, item i on spread stream i mod kSpread (with
+// `weight`, heaviest first, each on the stream with the least weight so
+// far): the
 // spread streams wait for the main stream's work so far, and the main stream
 // waits for all of them before anything issued after this call.  Items run
 // concurrently, so they must not share scratch memory (per-map work on
@@ -122,7 +124,8 @@ void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::fu
 // being captured; until then, for n <= 1 and while per-kernel timing is on,
 // the items run in order on the main stream.  Under graph capture they
 // become parallel branches.
-void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work);
+void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work,
+                const std::vector<double>* weight = nullptr);
 
 // Makes `ctx`'s device current for the lifetime of the guard.
 struct DeviceGuard {
