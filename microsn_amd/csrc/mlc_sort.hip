@@ -373,7 +373,14 @@ __global__ __launch_bounds__(kThreads, 3) void k_scatter(const uint64_t* __restr
 }
 
 // ---- sorts of up to kSmallSort keys: segments sorted in LDS, then merged.
-constexpr int kSegThreads = 256, kSegR = 8, kSeg = kSegThreads * kSegR;  // keys per segment
+// kSegThreads threads (kSegWaves waves) per segment of kSeg keys, kSegR keys
+// a thread: sixteen waves a CU cover the latency of each pass's LDS and
+// barrier chain, which one wave per SIMD left exposed.
+constexpr int kSegThreads = 1024, kSegR = 2, kSeg = kSegThreads * kSegR;  // keys per segment
+constexpr int kSegWaves = kSegThreads / 64;
+constexpr int kSegParts = kSegThreads / 256;  // threads per digit in the scan
+constexpr int kSegEnt = kSegR * kSegWaves;    // (round, wave) entries per digit
+static_assert(kSegEnt % kSegParts == 0, "the scan splits a digit's entries evenly");
 constexpr int kMergeThreads = 256, kMergeItems = 2;  // outputs per thread of a merge round
 constexpr int kMergeTile = kMergeThreads * kMergeItems;  // outputs per workgroup
 static_assert((2 * kSeg) % kMergeTile == 0, "a merge workgroup stays inside one pair of runs");
@@ -381,19 +388,22 @@ static_assert((2 * kSeg) % kMergeTile == 0, "a merge workgroup stays inside one 
 // Stable sort of one segment of kSeg keys (the doubles, turned into ordered
 // integer keys as k_transform_hist does), with their original indices as
 // payload, entirely in LDS: eight 8-bit LSD passes, each ranking keys stably
-// as k_scatter does (segment order = (round, thread)); a pass whose digit
-// is the same for every key of the segment is the identity and is skipped.
+// as k_scatter does (segment order = (round, wave, lane)); a pass whose
+// digit is the same for every key of the segment is the identity and is
+// skipped.
 __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const double* __restrict__ in, int64_t n,
                                                           uint64_t* __restrict__ kout,
                                                           unsigned* __restrict__ pout) {
   __shared__ uint64_t sk[kSeg];
   __shared__ unsigned sp[kSeg];
-  // cnt[r][w][d]: keys of digit d in round r of wave w; then, in place, the
-  // number of keys of digit d before that (round, wave) in segment order
-  __shared__ __align__(16) unsigned cnt[kSegR][kWaves][256];
-  __shared__ unsigned wsum[kWaves];
+  // cnt[e][d], e = round * kSegWaves + wave: keys of digit d in that (round,
+  // wave); then, in place, the number of keys of digit d before it in
+  // segment order
+  __shared__ __align__(16) unsigned cnt[kSegEnt][256];
+  __shared__ unsigned ptot[kSegParts][256];  // keys of digit d in each part's entries
+  __shared__ unsigned wsum[4];
   __shared__ unsigned start[256];  // first slot of each digit (exclusive scan)
-  __shared__ uint64_t red[2][kWaves];
+  __shared__ uint64_t red[2][kSegWaves];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t base = (int64_t)blockIdx.x * kSeg;
   const int valid = (int)min<int64_t>(kSeg, n - base);
@@ -419,18 +429,19 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const double* __restri
   uint64_t varying = 0;  // bits that differ between two keys of the segment
   {
     uint64_t a = ~0ull, o = 0ull;
-    for (int ww = 0; ww < kWaves; ++ww) {
+    for (int ww = 0; ww < kSegWaves; ++ww) {
       a &= red[0][ww];
       o |= red[1][ww];
     }
     varying = a ^ o;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  static_assert(kSegThreads == 256, "one thread per digit");
+  // the scan: thread t takes digit sd and entries [se0, se0 + kSegEnt/kSegParts)
+  const int sd = t & 255, sj = t >> 8, se0 = sj * (kSegEnt / kSegParts);
   for (int pass = 0; pass < 8; ++pass) {
     if (((varying >> (8 * pass)) & 255u) == 0) continue;  // block-uniform
-    for (int i = t; i < kSegR * kWaves * 64; i += kSegThreads)
-      reinterpret_cast<uint4*>(&cnt[0][0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = t; i < kSegEnt * 64; i += kSegThreads)
+      reinterpret_cast<uint4*>(&cnt[0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
     uint64_t key[kSegR];
     unsigned pay[kSegR], dig[kSegR], before[kSegR];
 #pragma unroll
@@ -454,37 +465,50 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(const double* __restri
       }
       if (d >= 256u) peers = 0;
       before[r] = (unsigned)__popcll(peers & lt);
-      if (d < 256u && before[r] == 0) cnt[r][w][d] = (unsigned)__popcll(peers);
+      if (d < 256u && before[r] == 0) cnt[r * kSegWaves + w][d] = (unsigned)__popcll(peers);
     }
     __syncthreads();
-    // thread t owns digit t: exclusive prefix over (round, wave), which is
-    // segment order (q = round * 256 + wave * 64 + lane), then the digit's
-    // start from a scan of the totals over the digits
-    unsigned tot = 0;
+    // per digit, kSegParts threads each sum a stretch of its entries ...
+    unsigned part = 0;
 #pragma unroll
-    for (int r = 0; r < kSegR; ++r)
+    for (int e = 0; e < kSegEnt / kSegParts; ++e) part += cnt[se0 + e][sd];
+    ptot[sj][sd] = part;
+    __syncthreads();
+    // ... then turn their stretch into exclusive prefixes in segment order;
+    // the first 256 threads also scan the digit totals
+    unsigned run = 0, tot = 0;
 #pragma unroll
-      for (int ww = 0; ww < kWaves; ++ww) {
-        const unsigned v = cnt[r][ww][t];
-        cnt[r][ww][t] = tot;
-        tot += v;
-      }
+    for (int j = 0; j < kSegParts; ++j) {
+      const unsigned v = ptot[j][sd];
+      if (j < sj) run += v;
+      tot += v;
+    }
+#pragma unroll
+    for (int e = 0; e < kSegEnt / kSegParts; ++e) {
+      const unsigned v = cnt[se0 + e][sd];
+      cnt[se0 + e][sd] = run;
+      run += v;
+    }
     unsigned incl = tot;
+    if (sj == 0) {
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const unsigned u = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += u;
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+      }
+      if (lane == 63) wsum[w] = incl;
     }
-    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    unsigned pre = 0;
-    for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
-    start[t] = pre + incl - tot;
+    if (sj == 0) {
+      unsigned pre = 0;
+      for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+      start[sd] = pre + incl - tot;
+    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSegR; ++r) {
       if (dig[r] < 256u) {
-        const unsigned q = start[dig[r]] + cnt[r][w][dig[r]] + before[r];
+        const unsigned q = start[dig[r]] + cnt[r * kSegWaves + w][dig[r]] + before[r];
         sk[q] = key[r];
         sp[q] = pay[r];
       }
