@@ -109,18 +109,22 @@ __global__ void k_sample_points(const T* __restrict__ conv, int Nx, int Ny, int 
 }
 
 namespace mlc {
-void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work, const std::vector<double>* weight) {
+void make_spread_streams(mlc_ctx* ctx) {
+  if (ctx->spread[0]) return;
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   hip_check(hipStreamIsCapturing(ctx->stream, &st), "hipStreamIsCapturing");
-  if (n > 1 && !ctx->spread[0] && st == hipStreamCaptureStatusNone) {
-    if (!ctx->fork_ev)
-      hip_check(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)");
-    for (int i = 0; i < mlc_ctx::kSpread; ++i) {
-      // default priority, like every other stream of the library
-      hip_check(hipStreamCreateWithFlags(&ctx->spread[i], hipStreamNonBlocking), "hipStreamCreateWithFlags(spread)");
-      hip_check(hipEventCreateWithFlags(&ctx->spread_ev[i], hipEventDisableTiming), "hipEventCreate(spread)");
-    }
+  if (st != hipStreamCaptureStatusNone) return;
+  if (!ctx->fork_ev)
+    hip_check(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)");
+  for (int i = 0; i < mlc_ctx::kSpread; ++i) {
+    // default priority, like every other stream of the library
+    hip_check(hipStreamCreateWithFlags(&ctx->spread[i], hipStreamNonBlocking), "hipStreamCreateWithFlags(spread)");
+    hip_check(hipEventCreateWithFlags(&ctx->spread_ev[i], hipEventDisableTiming), "hipEventCreate(spread)");
   }
+}
+
+void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work, const std::vector<double>* weight) {
+  if (n > 1) make_spread_streams(ctx);
   // with per-kernel timing on, in order: the kernels' own durations, not
   // those of kernels sharing the chip
   if (n <= 1 || !ctx->spread[0] || ctx->timing) {

@@ -127,6 +127,11 @@ void run_beside(mlc_ctx* ctx, const std::function<void()>& beside, const std::fu
 void run_spread(mlc_ctx* ctx, int n, const std::function<void(int)>& work,
                 const std::vector<double>* weight = nullptr);
 
+// Creates the spread streams and their events if they do not exist yet and
+// the context's stream is not being captured (run_spread does so itself;
+// this is for a caller that spreads its work only under capture).
+void make_spread_streams(mlc_ctx* ctx);
+
 // Makes `ctx`'s device current for the lifetime of the guard.
 struct DeviceGuard {
   int prev = -1;
