@@ -43,10 +43,12 @@ void fft_map_transform(mlc_map* m, const double* d_map);
 template <class T>
 void fft_convolve_fused(mlc_map* m, const ProfileParams& P, void* d_value, void* d_node);
 // Node maps of several profiles (see mlc_map_convolve_nodes).  Without a
-// driver the nodes are convolved in order; with one, drive(conv) is called
-// once after the profiles and taps of all nodes are built, and conv(k)
-// convolves node k when the driver calls it (each node once).
-using NodeDriver = std::function<void(const std::function<void(int)>& conv)>;
+// driver the nodes are convolved in order; with one, drive(conv, independent)
+// is called once after the profiles and taps of all nodes are built, and
+// conv(k) convolves node k when the driver calls it (each node once).
+// independent: no two nodes share a scratch buffer, so conv(k) of different
+// nodes may run on different streams.
+using NodeDriver = std::function<void(const std::function<void(int)>& conv, bool independent)>;
 template <class T>
 void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* const* d_nodes,
                         const NodeDriver* drive = nullptr);

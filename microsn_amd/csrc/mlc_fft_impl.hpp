@@ -1574,9 +1574,11 @@ bool node_rows(mlc_ctx* ctx, const cpx<T>* Hv, const cpx<T>* Hd, T* out, int Nx,
 // FP32 columns each workgroup filters four rows through its two buffers
 // (H = 2), so each row and tap load serves four output rows.  ldg and ldt
 // (row strides of Gs and tp) must be even, and the row spectrum below 4 GiB.
+// With launch = false nothing is launched: the return value says whether the
+// call would launch k_node_fused.
 template <class T>
 bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int ldt, int h, T* out, int Nx,
-                int Ny, const double* shift) {
+                int Ny, const double* shift, bool launch = true) {
   const int hp = (h + kYfK - 1) / kYfK * kYfK;
   if (sizeof(cpx<T>) * (size_t)(kYfRows + 3 * hp) * kYfCols > kLdsBytes) return false;
   if (Ny < h + 16) return false;
@@ -1589,13 +1591,14 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int l
     auto lds_for = [](int r) { return sizeof(cpx<T>) * (size_t)(Gm::NTW + r * Gm::LDS); };
     auto run = [&](auto rc, auto hc) {
       constexpr int R = decltype(rc)::value, H = decltype(hc)::value;
+      done = true;
+      if (!launch) return;
       const size_t lds = lds_for(R);
       auto k = k_node_fused<T, N, R, H>;
       allow_lds(k, lds);
       const int nq = (Ny + R * H - 1) / (R * H);
       MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(Gm::G * R), lds, Gs, ldg, tp, ldt,
                  h, cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, ct_twiddles<T, N, true>(ctx));
-      done = true;
     };
     using std::integral_constant;
     if constexpr (N <= 1024) {
@@ -1614,12 +1617,13 @@ bool node_fused(mlc_ctx* ctx, const cpx<T>* Gs, int ldg, const cpx<T>* tp, int l
     constexpr int R = sizeof(T) == 4 ? decltype(rc)::value : (decltype(rc)::value > 1 ? decltype(rc)::value / 2 : 1);
     const size_t lds = sizeof(cpx<T>) * ((size_t)N / 4 + (size_t)R * (N + N / 16));
     if (lds > kLdsBytes) return;
+    done = true;
+    if (!launch) return;
     auto k = k_node_fused_mix<T, N, R, G>;
     allow_lds(k, lds);
     const int nq = (Ny + R - 1) / R;
     MLC_LAUNCH(ctx, "conv_node_fused", k, dim3(((nq + 7) / 8) * 8), dim3(G * R), lds, Gs, ldg, tp, ldt, h,
                cos_table(ctx, 3, Nx), (cpx<T>*)out, Ny, shift, twiddles<T>(ctx, N));
-    done = true;
   };
   using std::integral_constant;
   switch (Nx) {
@@ -1881,7 +1885,7 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
   mlc_ctx* ctx = m->ctx;
   const int cnt = (int)Ps.size();
   if (cnt == 0) {
-    if (drive) (*drive)([](int) {});
+    if (drive) (*drive)([](int) {}, true);
     return;
   }
   std::vector<NodeJob> jobs;
@@ -1895,7 +1899,12 @@ void fft_convolve_nodes(mlc_map* m, const std::vector<ProfileParams>& Ps, void* 
     convolve_node<T>(m, jobs[k], d_nodes[k]);
   };
   if (drive) {
-    (*drive)(conv);
+    // every node by k_node_fused alone: the nodes share no scratch buffer
+    bool independent = true;
+    for (const NodeJob& J : jobs)
+      independent = independent && node_fused<T>(ctx, (const cpx<T>*)m->rows, m->ldr, static_cast<const cpx<T>*>(J.tp),
+                                                  J.ldt, J.P.n / 2, (T*)d_nodes[0], m->Nx, m->Ny, m->mean, false);
+    (*drive)(conv, independent);
   } else {
     for (int k = 0; k < cnt; ++k) conv(k);
   }

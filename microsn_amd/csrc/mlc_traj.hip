@@ -1632,9 +1632,18 @@ int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix, v
     // cache (convolving each node right before the band launch that reads
     // it let the band kernels evict the spectrum between nodes, and the step
     // took 3% longer); the curve partition runs beside the node kernels
-    NodeDriver drive = [&](const std::function<void(int)>& conv) {
+    NodeDriver drive = [&](const std::function<void(int)>& conv, bool independent) {
+      // under graph capture, independent nodes become parallel branches on
+      // the spread streams (the tail of each node kernel covered by
+      // another's); issued eagerly they stay in order
       const std::function<void()> nodes = [&] {
-        for (int k = 0; k < nnodes; ++k) conv(k);
+        const bool spread = independent && nnodes > 1;
+        if (spread && capturing(ctx)) {
+          run_spread(ctx, nnodes, conv);
+        } else {
+          if (spread) make_spread_streams(ctx);  // for a later capture
+          for (int k = 0; k < nnodes; ++k) conv(k);
+        }
       };
       run_fused(ctx, a, nullptr, &nodes);
     };
