@@ -305,7 +305,11 @@ int mlc_chi2_multiband(mlc_ctx* ctx, const float* d_f_obs, const float* d_df_obs
  *   params).  A curve leaving the pixel-centre hull or reaching mu <= 0 gets
  *   chi2 = +inf and a zero gradient.
  * Between radius nodes mu is the cubic Hermite interpolant of (mu, dmu/drad);
- * below the first / above the last node it is the first-order Taylor step. */
+ * below the first / above the last node it is the first-order Taylor step.
+ * Under graph capture the band launches of different profiles become
+ * parallel branches on four more streams of the context (default priority,
+ * created by an earlier call that was not captured); issued eagerly they run
+ * in order on the context's stream.  Results are the same bits either way. */
 int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes,
                      int nnodes, int Nx, int Ny, int P, int B,
                      const int* node_off, const int* node_cnt,
@@ -329,7 +333,9 @@ int mlc_lc_chi2_grad(mlc_ctx* ctx, int precision, const void* d_nodes,
  * partition runs on a second stream of the context (default priority,
  * created on the first call that is not being captured) beside the node
  * kernels, forked from and joined back into the context's stream by events
- * inside the call. */
+ * inside the call.  Under graph capture the node kernels too (when every
+ * node map comes from the fused kernel alone) are parallel branches on the
+ * four streams of mlc_lc_chi2_grad's band launches. */
 int mlc_lc_chi2_grad_convolve(mlc_map* map, const int* node_shape, double pix,
                               void* d_nodes, int nnodes, int P, int B,
                               const int* node_off, const int* node_cnt,
