@@ -46,9 +46,14 @@ def _step(ctx, w, convolve=False):
     return c, g, perm, sk, lo, npb, bc, be, w["nodes"].nodes
 
 
-@pytest.mark.parametrize("convolve,prec", [(False, "f32"), (True, "f32"), (True, "f64")])
-def test_captured_step_replays_match_eager(ctx, convolve, prec):
-    w = _workload(ctx, prec=prec)
+# Under capture the combined call's node kernels and the band launches of
+# the two profiles become parallel branches on the context's spread streams;
+# eager calls keep them in order.  1000^2: the node maps come from the fused
+# mixed-radix kernel, so they are branches as well.
+@pytest.mark.parametrize("convolve,prec,N", [(False, "f32", 512), (True, "f32", 512), (True, "f64", 512),
+                                             (True, "f32", 1000)])
+def test_captured_step_replays_match_eager(ctx, convolve, prec, N):
+    w = _workload(ctx, N=N, prec=prec)
     _step_c = lambda c, w: _step(c, w, convolve)
     eager = [t.clone() for t in _step_c(ctx, w)]
     torch.cuda.synchronize()
