@@ -1,7 +1,7 @@
 // mlc_oracle.cpp -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
 //
-// A plain restatement of the reference algorithm, used by tests/ as the
-// checker and by bench.py as the CPU baseline.  The product library never
+// A plain restatement of the reference algorithm, used by tests/ to check
+// results and by bench.py as the CPU baseline.  The product library never
 // links or calls this file.  Citations are relative to the reference root
 // (gvernard/microSN).  OpenMP is used only over independent rows/curves, so
 // results do not depend on the thread count.

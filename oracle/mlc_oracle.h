@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain C++ restatement of the
  * reference algorithm (gvernard/microSN, simulating_lcurves/ and chi2/),
- * used by tests/ as the checker and by bench.py as the CPU baseline.  The
+ * used by tests/ to check results and by bench.py as the CPU baseline.  The
  * product (microsn_amd/libmlc.so) never links, loads or calls it.
  *
  * Every function cites the reference file:line it follows.  Functions that

@@ -1,5 +1,5 @@
-"""Pins of the FP64 CPU oracle (oracle/), which the GPU tests use as their
-checker.  Each test checks an oracle function against something the oracle
+"""Pins of the FP64 CPU oracle (oracle/), against which the GPU tests check
+their results.  Each test checks an oracle function against something the oracle
 does not share code with: the reference's own C++ compiled from its sources
 (oracle/_ref: locations.cpp, mpd.cpp, integration.cpp), values derived by
 hand from the reference's inputs (grid_input.json), closed forms, numpy, or
